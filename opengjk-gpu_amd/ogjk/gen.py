@@ -1,0 +1,88 @@
+"""Synthetic batches from libogjk_gen.so (counter-based, deterministic).
+
+A batch holds 2n bodies in one AoS float32 pool; pair i is (body 2i, body 2i+1).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_LIB = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lib",
+                    "libogjk_gen.so")
+
+SPHERE, ELLIPSOID, NOISY_SPHERE = 0, 2, 3
+
+
+class _Spec(C.Structure):
+    _fields_ = [("family", C.c_int), ("seed", C.c_uint64), ("n_bodies", C.c_int),
+                ("nverts", C.c_int), ("nmin", C.c_int), ("nmax", C.c_int),
+                ("box", C.c_double), ("sigma", C.c_double)]
+
+
+_lib = None
+
+
+def _gen():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            raise RuntimeError(f"{_LIB} missing: run __graft_entry__.build()")
+        lib = C.CDLL(_LIB)
+        lib.ogjk_gen_counts.restype = C.c_int64
+        lib.ogjk_gen_counts.argtypes = [C.POINTER(_Spec), C.c_void_p]
+        lib.ogjk_gen_fill.restype = None
+        lib.ogjk_gen_fill.argtypes = [C.POINTER(_Spec), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        _lib = lib
+    return _lib
+
+
+@dataclass
+class Batch:
+    coords: np.ndarray  # (N, 3) float32 AoS pool
+    off: np.ndarray     # int64 [bodies], first vertex of each body
+    cnt: np.ndarray     # int32 [bodies]
+    pa: np.ndarray      # int32 [pairs]
+    pb: np.ndarray      # int32 [pairs]
+
+    @property
+    def n_pairs(self) -> int:
+        return len(self.pa)
+
+    def body(self, b: int) -> np.ndarray:
+        return self.coords[self.off[b]:self.off[b] + self.cnt[b]]
+
+
+def make(family: int, n_pairs: int, seed: int, nverts: int = 64, box: float = 2.0,
+         nmin: int = 4, nmax: int = 4096, sigma: float = 0.01, nthreads: int = 8) -> Batch:
+    lib = _gen()
+    nb = 2 * n_pairs
+    spec = _Spec(family, seed, nb, nverts, nmin, nmax, box, sigma)
+    cnt = np.zeros(nb, dtype=np.int32)
+    total = lib.ogjk_gen_counts(C.byref(spec), cnt.ctypes.data_as(C.c_void_p))
+    off = np.zeros(nb, dtype=np.int64)
+    if nb > 1:
+        np.cumsum(cnt[:-1], out=off[1:])
+    coords = np.empty((int(total), 3), dtype=np.float32)
+    lib.ogjk_gen_fill(C.byref(spec), cnt.ctypes.data_as(C.c_void_p), off.ctypes.data_as(C.c_void_p),
+                      coords.ctypes.data_as(C.c_void_p), int(nthreads))
+    pa = np.arange(0, nb, 2, dtype=np.int32)
+    pb = pa + 1
+    return Batch(coords, off, cnt, pa, pb)
+
+
+def from_bodies(bodies_a, bodies_b) -> Batch:
+    """Pack explicit vertex arrays (lists of (n,3) arrays) into a Batch."""
+    bodies = []
+    for a, b in zip(bodies_a, bodies_b):
+        bodies += [np.asarray(a, dtype=np.float32).reshape(-1, 3),
+                   np.asarray(b, dtype=np.float32).reshape(-1, 3)]
+    cnt = np.array([len(x) for x in bodies], dtype=np.int32)
+    off = np.zeros(len(bodies), dtype=np.int64)
+    if len(bodies) > 1:
+        np.cumsum(cnt[:-1], out=off[1:])
+    coords = np.concatenate(bodies, axis=0) if bodies else np.zeros((0, 3), np.float32)
+    pa = np.arange(0, len(bodies), 2, dtype=np.int32)
+    return Batch(np.ascontiguousarray(coords), off, cnt, pa, pa + 1)

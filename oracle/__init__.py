@@ -1,0 +1,161 @@
+"""Oracle bindings (test infrastructure only).
+
+Loads the CPU restatement of the reference GJK/EPA (``oracle/lib``) and, when it
+has been built in a container that holds the reference checkout, the reference
+CPU sources themselves (``oracle/_ref/libref32.so``, FP32 as shipped).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the benchmark's CPU-baseline leg
+use this module, always as the checker.  The shipped library never calls it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB64 = os.path.join(_HERE, "lib", "liboracle64.so")
+LIB32 = os.path.join(_HERE, "lib", "liboracle32.so")
+REF32 = os.path.join(_HERE, "_ref", "libref32.so")
+
+GJK, EPA, GJK_EPA = 1, 2, 3
+
+
+def simplex_dtype(bits: int) -> np.dtype:
+    """numpy view of gkSimplex (GJK/common.h:84-89) at the given precision."""
+    real = np.float64 if bits == 64 else np.float32
+    offs = [0, 8, 104, 136] if bits == 64 else [0, 4, 52, 84]
+    return np.dtype({
+        "names": ["nvrtx", "vrtx", "vrtx_idx", "witnesses"],
+        "formats": [np.int32, (real, (4, 3)), (np.int32, (4, 2)), (real, (2, 3))],
+        "offsets": offs,
+        "itemsize": 184 if bits == 64 else 108,
+    })
+
+
+STATS_DTYPE = np.dtype([("gjk_iters", np.int32), ("epa_iters", np.int32),
+                        ("sweeps_a", np.int32), ("sweeps_b", np.int32)])
+
+_libs: dict = {}
+
+
+def _lib(bits: int):
+    if bits not in _libs:
+        path = LIB64 if bits == 64 else LIB32
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: {path} (run __graft_entry__.build())")
+        lib = C.CDLL(path)
+        lib.oracle_batch.restype = None
+        lib.oracle_batch.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 9 + [C.c_int]
+        lib.oracle_real_size.restype = C.c_int
+        assert lib.oracle_real_size() == bits // 8
+        _libs[bits] = lib
+    return _libs[bits]
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+@dataclass
+class Result:
+    distances: np.ndarray
+    simplices: np.ndarray
+    normals: np.ndarray | None
+    stats: np.ndarray | None
+
+
+def run(mode: int, coords, off, cnt, pa, pb, bits: int = 64, nthreads: int = 1,
+        simplices=None, distances=None, stats: bool = False) -> Result:
+    """Run the oracle over pairs (pa[i], pb[i]) of bodies in an AoS pool.
+
+    ``coords`` is an (N, 3) float32 pool; it is promoted exactly to the oracle's
+    precision.  For mode EPA, ``simplices``/``distances`` are the inputs (copied).
+    """
+    real = np.float64 if bits == 64 else np.float32
+    n = len(pa)
+    c = np.ascontiguousarray(coords, dtype=real).reshape(-1, 3)
+    off = np.ascontiguousarray(off, dtype=np.int64)
+    cnt = np.ascontiguousarray(cnt, dtype=np.int32)
+    pa = np.ascontiguousarray(pa, dtype=np.int32)
+    pb = np.ascontiguousarray(pb, dtype=np.int32)
+    sd = simplex_dtype(bits)
+    if simplices is None:
+        simp = np.zeros(n, dtype=sd)
+    else:
+        simp = np.array(simplices, dtype=sd, copy=True)
+    dist = np.zeros(n, dtype=real) if distances is None else np.array(distances, dtype=real, copy=True)
+    normals = np.zeros((n, 3), dtype=real) if mode & EPA else None
+    st = np.zeros(n, dtype=STATS_DTYPE) if stats else None
+    if n:
+        _lib(bits).oracle_batch(mode, n, _p(c), _p(off), _p(cnt), _p(pa), _p(pb), _p(simp),
+                                _p(dist), _p(normals), _p(st), int(nthreads))
+    return Result(dist, simp, normals, st)
+
+
+# ----------------------------------------------------------------- reference
+
+class _Poly32(C.Structure):
+    _fields_ = [("numpoints", C.c_int), ("s", C.c_float * 3), ("s_idx", C.c_int),
+                ("coord", C.POINTER(C.c_float))]
+
+
+class _Simplex32(C.Structure):
+    _fields_ = [("nvrtx", C.c_int), ("vrtx", (C.c_float * 3) * 4), ("vrtx_idx", (C.c_int * 2) * 4),
+                ("witnesses", (C.c_float * 3) * 2)]
+
+
+assert C.sizeof(_Poly32) == 32 and C.sizeof(_Simplex32) == 108
+
+_ref = None
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF32)
+
+
+def _ref_lib():
+    global _ref
+    if _ref is None:
+        lib = C.CDLL(REF32)
+        lib.compute_minimum_distance.restype = C.c_float
+        lib.compute_minimum_distance.argtypes = [_Poly32, _Poly32, C.POINTER(_Simplex32)]
+        lib.computeCollisionInformation.restype = None
+        lib.computeCollisionInformation.argtypes = [C.POINTER(_Poly32), C.POINTER(_Poly32),
+                                                    C.POINTER(_Simplex32), C.POINTER(C.c_float),
+                                                    C.POINTER(C.c_float)]
+        _ref = lib
+    return _ref
+
+
+def ref_run(mode: int, coords, off, cnt, pa, pb) -> Result:
+    """Reference CPU sources (FP32 as shipped), one call per pair.
+
+    Mirrors GJK::CPU::computeDistances / computeEPA (examples/cpu/example.cpp:13-40).
+    """
+    lib = _ref_lib()
+    c = np.ascontiguousarray(coords, dtype=np.float32).reshape(-1, 3)
+    base = c.ctypes.data
+    n = len(pa)
+    sd = simplex_dtype(32)
+    out = np.zeros(n, dtype=sd)
+    dist = np.zeros(n, dtype=np.float32)
+    normals = np.zeros((n, 3), dtype=np.float32) if mode & EPA else None
+    fp = C.POINTER(C.c_float)
+    for i in range(n):
+        a, b = int(pa[i]), int(pb[i])
+        p1 = _Poly32(int(cnt[a]), (C.c_float * 3)(), 0, C.cast(base + 12 * int(off[a]), fp))
+        p2 = _Poly32(int(cnt[b]), (C.c_float * 3)(), 0, C.cast(base + 12 * int(off[b]), fp))
+        s = _Simplex32()
+        d = C.c_float(0.0)
+        if mode & GJK:
+            d.value = lib.compute_minimum_distance(p1, p2, C.byref(s))
+        if mode & EPA:
+            nrm = (C.c_float * 3)()
+            lib.computeCollisionInformation(C.byref(p1), C.byref(p2), C.byref(s), C.byref(d), nrm)
+            normals[i] = np.frombuffer(nrm, dtype=np.float32)
+        dist[i] = d.value
+        out[i] = np.frombuffer(bytes(s), dtype=sd)[0]
+    return Result(dist, out, normals, None)
