@@ -1,0 +1,136 @@
+/*
+ * opengjk-amd C ABI (libogjk.so): batched GJK distance and EPA penetration
+ * for pairs of convex polytopes on AMD Instinct MI355X (gfx950).
+ *
+ * Plain pointers and sizes only.  Every entry point returns an ogjk_status;
+ * ogjk_last_error() gives the calling thread's last message.  Results are
+ * those of the FP64 reference CPU algorithm (GJK/cpu/openGJK.c, GJK/cpu/EPA.c)
+ * rounded once to the gkFloat ABI.
+ *
+ * Mode (what a call computes), mirroring the reference entry points:
+ *   OGJK_MODE_GJK      distance + simplex + witnesses
+ *                      (compute_minimum_distance_kernel, GJK/gpu/openGJK.cu:1427)
+ *   OGJK_MODE_EPA      EPA only; reads simplices/distances written by GJK
+ *                      (compute_epa_kernel, GJK/gpu/openGJK.cu:2729)
+ *   OGJK_MODE_GJK_EPA  both, fused in one kernel (GJK::GPU::computeGJKAndEPA,
+ *                      examples/gpu/example.cu:54-84)
+ *
+ * Output conventions (GJK/gpu/openGJK.h:64-119): distance >= 0 for separated
+ * pairs; EPA writes distance = -depth for penetrating pairs, overwrites
+ * simplices[i].witnesses (witness on body 1, on body 2) and writes the unit
+ * contact normal (3 floats per pair, AoS).  A pair with a body of fewer than
+ * one vertex or an out-of-range pair index gets distance NaN, nvrtx 0.
+ */
+#ifndef OPENGJK_AMD_OGJK_H
+#define OPENGJK_AMD_OGJK_H
+
+#include <stdint.h>
+
+#include "opengjk/common.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum ogjk_status {
+  OGJK_OK = 0,
+  OGJK_ERR_INVALID = 1, /* bad argument */
+  OGJK_ERR_HIP = 2,     /* HIP runtime error (message in ogjk_last_error) */
+  OGJK_ERR_NOMEM = 3,   /* device or host allocation failed */
+  OGJK_ERR_UNSUPPORTED = 4
+} ogjk_status;
+
+enum { OGJK_MODE_GJK = 1, OGJK_MODE_EPA = 2, OGJK_MODE_GJK_EPA = 3 };
+
+/* Device-resident structure-of-arrays scene.  Body b owns vertices
+ * [body_off[b], body_off[b] + body_cnt[b]) of x/y/z; body_r1[b] >=
+ * max |x|+|y|+|z| over its vertices (written by ogjk_pack_scene). */
+typedef struct ogjk_scene {
+  const float* x;
+  const float* y;
+  const float* z;
+  const int64_t* body_off;
+  const int32_t* body_cnt;
+  const float* body_r1;
+  int32_t n_bodies;
+} ogjk_scene;
+
+/* Plain-FP64 scene (double coordinates) for the comparison path. */
+typedef struct ogjk_scene64 {
+  const double* x;
+  const double* y;
+  const double* z;
+  const int64_t* body_off;
+  const int32_t* body_cnt;
+  int32_t n_bodies;
+} ogjk_scene64;
+
+/* Per-pair FP64 record for verification (optional, may be NULL):
+ * [distance, w0.xyz, w1.xyz, normal.xyz, vrtx[0..3].xyz, 2 unused] */
+enum { OGJK_F64_RECORD = 24 };
+
+/* ------------------------------------------------------------ library */
+
+int ogjk_version(void);
+const char* ogjk_last_error(void);
+/* Number of visible HIP devices (0 without a GPU; never fails). */
+int ogjk_device_count(void);
+ogjk_status ogjk_sync(void* stream);
+
+/* ------------------------------------------------------------ scenes */
+
+/* Pack an AoS float pool (device) into the SoA scene arrays (device) and
+ * compute body_r1.  n_verts = total vertices. */
+ogjk_status ogjk_pack_scene(const float* d_coords_aos, const int64_t* d_body_off,
+                            const int32_t* d_body_cnt, int32_t n_bodies, int64_t n_verts,
+                            float* d_x, float* d_y, float* d_z, float* d_r1, void* stream);
+
+/* Pairs over a resident scene; d_pairs holds 2 int32 per pair (idx1, idx2)
+ * or is NULL for the implicit pairs (2i, 2i+1).  Asynchronous on `stream`.
+ * Replaces compute_minimum_distance_indexed_device / compute_epa_indexed_device
+ * (GJK/gpu/openGJK.h:419-457) for SoA scenes. */
+ogjk_status ogjk_collide_scene(int mode, const ogjk_scene* scene, const int32_t* d_pairs,
+                               int32_t n_pairs, gkSimplex* d_simplices, float* d_distances,
+                               float* d_normals, double* d_f64, void* stream);
+
+/* Plain-FP64 comparison path: double storage, FP64 sweep, no candidate
+ * re-rank; same selection rules and results (the denominator of the
+ * mixed-precision speed-up). */
+ogjk_status ogjk_pack_scene_f64(const float* d_coords_aos, int64_t n_verts, double* d_x,
+                                double* d_y, double* d_z, void* stream);
+ogjk_status ogjk_collide_scene_f64(int mode, const ogjk_scene64* scene, const int32_t* d_pairs,
+                                   int32_t n_pairs, gkSimplex* d_simplices, float* d_distances,
+                                   float* d_normals, double* d_f64, void* stream);
+
+/* ------------------------------------------------------------ gkPolytope input */
+
+/* Device gkPolytope arrays whose coord are device AoS pointers; pair i is
+ * (d_bd1[i], d_bd2[i]).  Replaces compute_minimum_distance_device and
+ * compute_epa_device (GJK/gpu/openGJK.h:237-262).  Asynchronous. */
+ogjk_status ogjk_collide_polytopes(int mode, const gkPolytope* d_bd1, const gkPolytope* d_bd2,
+                                   int32_t n, gkSimplex* d_simplices, float* d_distances,
+                                   float* d_normals, void* stream);
+
+/* Indexed form over a device polytope pool (GJK/gpu/openGJK.h:398-505). */
+ogjk_status ogjk_collide_polytopes_indexed(int mode, const gkPolytope* d_pool,
+                                           int32_t n_polytopes, const gkCollisionPair* d_pairs,
+                                           int32_t n_pairs, gkSimplex* d_simplices,
+                                           float* d_distances, float* d_normals, void* stream);
+
+/* Host arrays in, host arrays out, synchronous; device memory is staged
+ * internally.  Replaces compute_minimum_distance / computeCollisionInformation
+ * / compute_gjk_epa (GJK/gpu/openGJK.h:91-141).  normals may be NULL for GJK. */
+ogjk_status ogjk_collide_host(int mode, int32_t n, const gkPolytope* bd1, const gkPolytope* bd2,
+                              gkSimplex* simplices, float* distances, float* normals);
+
+/* Host indexed form: compute_minimum_distance_indexed / compute_epa_indexed /
+ * compute_gjk_epa_indexed (GJK/gpu/openGJK.h:398-505). */
+ogjk_status ogjk_collide_host_indexed(int mode, int32_t n_polytopes, int32_t n_pairs,
+                                      const gkPolytope* polytopes, const gkCollisionPair* pairs,
+                                      gkSimplex* simplices, float* distances, float* normals);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPENGJK_AMD_OGJK_H */

@@ -1,0 +1,296 @@
+// opengjk-amd C ABI: argument checking, host staging and launches.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "ogjk_internal.hpp"
+#include "opengjk/ogjk.h"
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+ogjk_status fail(ogjk_status st, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return st;
+}
+
+ogjk_status hip_fail(hipError_t e, const char* what) {
+  return fail(e == hipErrorOutOfMemory ? OGJK_ERR_NOMEM : OGJK_ERR_HIP, "%s: %s", what,
+              hipGetErrorString(e));
+}
+
+#define OGJK_HIP(call, what)                     \
+  do {                                           \
+    const hipError_t e_ = (call);                \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+
+bool valid_mode(int m) { return m == OGJK_MODE_GJK || m == OGJK_MODE_EPA || m == OGJK_MODE_GJK_EPA; }
+
+// One device allocation carved into 256-byte aligned pieces, freed on scope exit.
+struct DeviceArena {
+  char* base = nullptr;
+  size_t used = 0, cap = 0;
+  ~DeviceArena() {
+    if (base) (void)hipFree(base);
+  }
+  static size_t up(size_t b) { return (b + 255) & ~size_t(255); }
+  hipError_t reserve(size_t bytes) {
+    cap = bytes;
+    return bytes ? hipMalloc((void**)&base, bytes) : hipSuccess;
+  }
+  template <class T>
+  T* take(size_t count) {
+    T* p = (T*)(base + used);
+    used += up(count * sizeof(T));
+    return p;
+  }
+};
+
+// Host SoA staging of a list of gkPolytope, with per-body L1 bound.
+struct HostScene {
+  std::vector<float> x, y, z, r1;
+  std::vector<int64_t> off;
+  std::vector<int32_t> cnt;
+};
+
+ogjk_status stage_bodies(const gkPolytope* const* lists, const int* counts, int nlists,
+                         bool interleave, HostScene& hs) {
+  // interleave: body 2i = lists[0][i], 2i+1 = lists[1][i] (implicit pairs)
+  int64_t nb = 0, nv = 0;
+  for (int l = 0; l < nlists; l++) nb += counts[l];
+  hs.off.resize(nb);
+  hs.cnt.resize(nb);
+  hs.r1.resize(nb);
+  auto body_at = [&](int64_t b) -> const gkPolytope& {
+    if (interleave) return lists[b & 1][b >> 1];
+    return lists[0][b];
+  };
+  for (int64_t b = 0; b < nb; b++) {
+    const gkPolytope& p = body_at(b);
+    if (p.numpoints < 0) return fail(OGJK_ERR_INVALID, "body %lld: numpoints %d < 0", (long long)b, p.numpoints);
+    if (p.numpoints > 0 && p.coord == nullptr)
+      return fail(OGJK_ERR_INVALID, "body %lld: coord is NULL", (long long)b);
+    hs.off[b] = nv;
+    hs.cnt[b] = p.numpoints;
+    nv += p.numpoints;
+  }
+  hs.x.resize(nv);
+  hs.y.resize(nv);
+  hs.z.resize(nv);
+  for (int64_t b = 0; b < nb; b++) {
+    const gkPolytope& p = body_at(b);
+    const int64_t o = hs.off[b];
+    float m = 0.0f;
+    for (int i = 0; i < p.numpoints; i++) {
+      const float vx = p.coord[3 * i], vy = p.coord[3 * i + 1], vz = p.coord[3 * i + 2];
+      hs.x[o + i] = vx;
+      hs.y[o + i] = vy;
+      hs.z[o + i] = vz;
+      m = std::fmax(m, std::fabs(vx) + std::fabs(vy) + std::fabs(vz));
+    }
+    hs.r1[b] = m * (1.0f + 0x1.0p-22f);
+  }
+  return OGJK_OK;
+}
+
+// Upload a staged scene, run `mode` over `n` pairs and copy results back.
+ogjk_status run_staged(int mode, const HostScene& hs, const gkCollisionPair* pairs, int n,
+                       gkSimplex* simplices, float* distances, float* normals) {
+  const size_t nv = hs.x.size(), nb = hs.cnt.size();
+  DeviceArena ar;
+  size_t bytes = 3 * DeviceArena::up(nv * sizeof(float)) + DeviceArena::up(nb * 8) +
+                 2 * DeviceArena::up(nb * 4) + DeviceArena::up((size_t)n * sizeof(gkSimplex)) +
+                 DeviceArena::up((size_t)n * 4) + DeviceArena::up((size_t)n * 12) +
+                 DeviceArena::up((size_t)n * 8) + 256;
+  OGJK_HIP(ar.reserve(bytes), "hipMalloc(staging)");
+  float* dx = ar.take<float>(nv);
+  float* dy = ar.take<float>(nv);
+  float* dz = ar.take<float>(nv);
+  int64_t* doff = ar.take<int64_t>(nb);
+  int32_t* dcnt = ar.take<int32_t>(nb);
+  float* dr1 = ar.take<float>(nb);
+  gkSimplex* ds = ar.take<gkSimplex>(n);
+  float* dd = ar.take<float>(n);
+  float* dn = ar.take<float>(3 * (size_t)n);
+  int32_t* dp = pairs ? ar.take<int32_t>(2 * (size_t)n) : nullptr;
+
+  OGJK_HIP(hipMemcpy(dx, hs.x.data(), nv * 4, hipMemcpyHostToDevice), "hipMemcpy(x)");
+  OGJK_HIP(hipMemcpy(dy, hs.y.data(), nv * 4, hipMemcpyHostToDevice), "hipMemcpy(y)");
+  OGJK_HIP(hipMemcpy(dz, hs.z.data(), nv * 4, hipMemcpyHostToDevice), "hipMemcpy(z)");
+  OGJK_HIP(hipMemcpy(doff, hs.off.data(), nb * 8, hipMemcpyHostToDevice), "hipMemcpy(off)");
+  OGJK_HIP(hipMemcpy(dcnt, hs.cnt.data(), nb * 4, hipMemcpyHostToDevice), "hipMemcpy(cnt)");
+  OGJK_HIP(hipMemcpy(dr1, hs.r1.data(), nb * 4, hipMemcpyHostToDevice), "hipMemcpy(r1)");
+  if (pairs) OGJK_HIP(hipMemcpy(dp, pairs, (size_t)n * 8, hipMemcpyHostToDevice), "hipMemcpy(pairs)");
+  if (mode == OGJK_MODE_EPA) {
+    OGJK_HIP(hipMemcpy(ds, simplices, (size_t)n * sizeof(gkSimplex), hipMemcpyHostToDevice),
+             "hipMemcpy(simplices)");
+    OGJK_HIP(hipMemcpy(dd, distances, (size_t)n * 4, hipMemcpyHostToDevice), "hipMemcpy(distances)");
+  }
+  const ogjk::SceneArgs sa{dx, dy, dz, doff, dcnt, dr1, (int32_t)nb, dp};
+  const ogjk::OutArgs oa{ds, dd, dn, nullptr};
+  OGJK_HIP(ogjk::launch_scene(mode, sa, oa, n, nullptr), "kernel launch");
+  OGJK_HIP(hipDeviceSynchronize(), "kernel");
+  OGJK_HIP(hipMemcpy(simplices, ds, (size_t)n * sizeof(gkSimplex), hipMemcpyDeviceToHost),
+           "hipMemcpy(simplices)");
+  OGJK_HIP(hipMemcpy(distances, dd, (size_t)n * 4, hipMemcpyDeviceToHost), "hipMemcpy(distances)");
+  if ((mode & OGJK_MODE_EPA) && normals)
+    OGJK_HIP(hipMemcpy(normals, dn, (size_t)n * 12, hipMemcpyDeviceToHost), "hipMemcpy(normals)");
+  return OGJK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ogjk_version(void) { return 100; }
+
+const char* ogjk_last_error(void) { return g_err; }
+
+int ogjk_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+ogjk_status ogjk_sync(void* stream) {
+  OGJK_HIP(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+  return OGJK_OK;
+}
+
+ogjk_status ogjk_pack_scene(const float* d_coords_aos, const int64_t* d_body_off,
+                            const int32_t* d_body_cnt, int32_t n_bodies, int64_t n_verts,
+                            float* d_x, float* d_y, float* d_z, float* d_r1, void* stream) {
+  if (n_bodies < 0 || n_verts < 0) return fail(OGJK_ERR_INVALID, "negative size");
+  if (n_verts > 0 && (!d_coords_aos || !d_x || !d_y || !d_z))
+    return fail(OGJK_ERR_INVALID, "NULL coordinate pointer");
+  if (n_bodies > 0 && (!d_body_off || !d_body_cnt || !d_r1))
+    return fail(OGJK_ERR_INVALID, "NULL body table pointer");
+  OGJK_HIP(ogjk::launch_pack(d_coords_aos, d_body_off, d_body_cnt, n_bodies, n_verts, d_x, d_y,
+                             d_z, d_r1, (hipStream_t)stream),
+           "pack launch");
+  return OGJK_OK;
+}
+
+ogjk_status ogjk_pack_scene_f64(const float* d_coords_aos, int64_t n_verts, double* d_x,
+                                double* d_y, double* d_z, void* stream) {
+  if (n_verts < 0) return fail(OGJK_ERR_INVALID, "negative size");
+  if (n_verts > 0 && (!d_coords_aos || !d_x || !d_y || !d_z))
+    return fail(OGJK_ERR_INVALID, "NULL coordinate pointer");
+  OGJK_HIP(ogjk::launch_pack_f64(d_coords_aos, n_verts, d_x, d_y, d_z, (hipStream_t)stream),
+           "pack launch");
+  return OGJK_OK;
+}
+
+static ogjk_status check_out(int mode, int32_t n, gkSimplex* s, float* d, float* nrm) {
+  if (!valid_mode(mode)) return fail(OGJK_ERR_INVALID, "mode %d is not 1, 2 or 3", mode);
+  if (n < 0) return fail(OGJK_ERR_INVALID, "n = %d < 0", n);
+  if (n > 0 && (!s || !d)) return fail(OGJK_ERR_INVALID, "NULL simplices/distances");
+  (void)nrm;
+  return OGJK_OK;
+}
+
+ogjk_status ogjk_collide_scene(int mode, const ogjk_scene* scene, const int32_t* d_pairs,
+                               int32_t n_pairs, gkSimplex* d_simplices, float* d_distances,
+                               float* d_normals, double* d_f64, void* stream) {
+  const ogjk_status st = check_out(mode, n_pairs, d_simplices, d_distances, d_normals);
+  if (st != OGJK_OK) return st;
+  if (n_pairs == 0) return OGJK_OK;
+  if (!scene || !scene->x || !scene->y || !scene->z || !scene->body_off || !scene->body_cnt ||
+      !scene->body_r1)
+    return fail(OGJK_ERR_INVALID, "incomplete scene");
+  if (!d_pairs && (int64_t)scene->n_bodies < 2 * (int64_t)n_pairs)
+    return fail(OGJK_ERR_INVALID, "implicit pairs need %lld bodies, scene has %d",
+                2 * (long long)n_pairs, scene->n_bodies);
+  const ogjk::SceneArgs sa{scene->x, scene->y, scene->z, scene->body_off, scene->body_cnt,
+                           scene->body_r1, scene->n_bodies, d_pairs};
+  const ogjk::OutArgs oa{d_simplices, d_distances, d_normals, d_f64};
+  OGJK_HIP(ogjk::launch_scene(mode, sa, oa, n_pairs, (hipStream_t)stream), "kernel launch");
+  return OGJK_OK;
+}
+
+ogjk_status ogjk_collide_scene_f64(int mode, const ogjk_scene64* scene, const int32_t* d_pairs,
+                                   int32_t n_pairs, gkSimplex* d_simplices, float* d_distances,
+                                   float* d_normals, double* d_f64, void* stream) {
+  const ogjk_status st = check_out(mode, n_pairs, d_simplices, d_distances, d_normals);
+  if (st != OGJK_OK) return st;
+  if (n_pairs == 0) return OGJK_OK;
+  if (!scene || !scene->x || !scene->y || !scene->z || !scene->body_off || !scene->body_cnt)
+    return fail(OGJK_ERR_INVALID, "incomplete scene");
+  if (!d_pairs && (int64_t)scene->n_bodies < 2 * (int64_t)n_pairs)
+    return fail(OGJK_ERR_INVALID, "implicit pairs need %lld bodies, scene has %d",
+                2 * (long long)n_pairs, scene->n_bodies);
+  const ogjk::SceneArgs sa{scene->x, scene->y, scene->z, scene->body_off, scene->body_cnt,
+                           nullptr, scene->n_bodies, d_pairs};
+  const ogjk::OutArgs oa{d_simplices, d_distances, d_normals, d_f64};
+  OGJK_HIP(ogjk::launch_scene_f64(mode, sa, oa, n_pairs, (hipStream_t)stream), "kernel launch");
+  return OGJK_OK;
+}
+
+ogjk_status ogjk_collide_polytopes(int mode, const gkPolytope* d_bd1, const gkPolytope* d_bd2,
+                                   int32_t n, gkSimplex* d_simplices, float* d_distances,
+                                   float* d_normals, void* stream) {
+  const ogjk_status st = check_out(mode, n, d_simplices, d_distances, d_normals);
+  if (st != OGJK_OK) return st;
+  if (n == 0) return OGJK_OK;
+  if (!d_bd1 || !d_bd2) return fail(OGJK_ERR_INVALID, "NULL polytope array");
+  const ogjk::PolyArgs pa{d_bd1, d_bd2, nullptr, 0};
+  const ogjk::OutArgs oa{d_simplices, d_distances, d_normals, nullptr};
+  OGJK_HIP(ogjk::launch_poly(mode, pa, oa, n, (hipStream_t)stream), "kernel launch");
+  return OGJK_OK;
+}
+
+ogjk_status ogjk_collide_polytopes_indexed(int mode, const gkPolytope* d_pool,
+                                           int32_t n_polytopes, const gkCollisionPair* d_pairs,
+                                           int32_t n_pairs, gkSimplex* d_simplices,
+                                           float* d_distances, float* d_normals, void* stream) {
+  const ogjk_status st = check_out(mode, n_pairs, d_simplices, d_distances, d_normals);
+  if (st != OGJK_OK) return st;
+  if (n_pairs == 0) return OGJK_OK;
+  if (!d_pool || !d_pairs || n_polytopes <= 0)
+    return fail(OGJK_ERR_INVALID, "NULL pool/pairs or empty pool");
+  const ogjk::PolyArgs pa{d_pool, nullptr, d_pairs, n_polytopes};
+  const ogjk::OutArgs oa{d_simplices, d_distances, d_normals, nullptr};
+  OGJK_HIP(ogjk::launch_poly(mode, pa, oa, n_pairs, (hipStream_t)stream), "kernel launch");
+  return OGJK_OK;
+}
+
+ogjk_status ogjk_collide_host(int mode, int32_t n, const gkPolytope* bd1, const gkPolytope* bd2,
+                              gkSimplex* simplices, float* distances, float* normals) {
+  const ogjk_status st = check_out(mode, n, simplices, distances, normals);
+  if (st != OGJK_OK) return st;
+  if (n == 0) return OGJK_OK;
+  if (!bd1 || !bd2) return fail(OGJK_ERR_INVALID, "NULL polytope array");
+  const gkPolytope* lists[2] = {bd1, bd2};
+  const int counts[2] = {n, n};
+  HostScene hs;
+  const ogjk_status s2 = stage_bodies(lists, counts, 2, true, hs);
+  if (s2 != OGJK_OK) return s2;
+  return run_staged(mode, hs, nullptr, n, simplices, distances, normals);
+}
+
+ogjk_status ogjk_collide_host_indexed(int mode, int32_t n_polytopes, int32_t n_pairs,
+                                      const gkPolytope* polytopes, const gkCollisionPair* pairs,
+                                      gkSimplex* simplices, float* distances, float* normals) {
+  const ogjk_status st = check_out(mode, n_pairs, simplices, distances, normals);
+  if (st != OGJK_OK) return st;
+  if (n_pairs == 0) return OGJK_OK;
+  if (!polytopes || !pairs || n_polytopes <= 0)
+    return fail(OGJK_ERR_INVALID, "NULL polytopes/pairs or empty pool");
+  const gkPolytope* lists[1] = {polytopes};
+  const int counts[1] = {n_polytopes};
+  HostScene hs;
+  const ogjk_status s2 = stage_bodies(lists, counts, 1, false, hs);
+  if (s2 != OGJK_OK) return s2;
+  return run_staged(mode, hs, pairs, n_pairs, simplices, distances, normals);
+}
+
+}  // extern "C"

@@ -1,0 +1,685 @@
+// opengjk-amd device core: mixed-precision support sweep, GJK and EPA for one
+// pair on a group of LPP lanes.
+//
+// Precision design
+//   * Vertices are stored as FP32.  The O(n) support sweep runs in FP32
+//     (explicit FMAs) over an exactly power-of-two-scaled copy of the
+//     direction, tracking each lane's best and second-best value.
+//   * A rigorous bound on |FP32 dot - FP64 dot| (16 ulp of the direction's
+//     max-norm times the body's max L1 vertex norm) defines a candidate
+//     window below the FP32 maximum.  Only candidates are re-evaluated in
+//     FP64 with the reference's association, so the selected vertex is exactly
+//     the one the FP64 reference support selects, ties included.
+//   * Everything that is O(1) per iteration (simplex, sub-distance regions,
+//     termination, witnesses, EPA faces and horizon) runs in FP64 in the
+//     reference's operation order, without FMA contraction.
+//   Result: outputs match the FP64 CPU reference bit for bit before the final
+//   rounding to the gkFloat ABI.
+#pragma once
+
+#include "ogjk_math.hpp"
+
+namespace ogjk {
+
+// ------------------------------------------------------------------ bodies
+
+// Vertex i of a body lives at x[i*STRIDE], y[i*STRIDE], z[i*STRIDE]:
+// STRIDE 1 is the SoA scene, STRIDE 3 reads caller AoS coordinates in place.
+// T is the storage type: float (the product path) or double (plain FP64).
+template <class T, int STRIDE>
+struct Body {
+  const T* x;
+  const T* y;
+  const T* z;
+  int n;
+  float r1;  // >= max_i |x_i|+|y_i|+|z_i| (used by the float sweep only)
+  OGJK_DEV T rx(int i) const { return x[(int64_t)i * STRIDE]; }
+  OGJK_DEV T ry(int i) const { return y[(int64_t)i * STRIDE]; }
+  OGJK_DEV T rz(int i) const { return z[(int64_t)i * STRIDE]; }
+  OGJK_DEV double c(int i, int k) const { return (double)(k == 0 ? rx(i) : (k == 1 ? ry(i) : rz(i))); }
+  OGJK_DEV D3 d(int i) const { return D3{(double)rx(i), (double)ry(i), (double)rz(i)}; }
+};
+
+// Max L1 vertex norm over the group (FP32, rounded up by a relative 2^-22).
+template <int LPP, class T, int STRIDE>
+OGJK_DEV float body_r1(const Body<T, STRIDE>& b, int lane) {
+  float m = 0.0f;
+  for (int i = lane; i < b.n; i += LPP) {
+    const float s = fabsf((float)b.rx(i)) + fabsf((float)b.ry(i)) + fabsf((float)b.rz(i));
+    m = fmaxf(m, s);
+  }
+  m = group_max_f<LPP>(m);
+  return m * (1.0f + 0x1.0p-22f);
+}
+
+// --------------------------------------------------------------- support
+
+// Exact emulation of the FP64 reference argmax of dot(x_i, dir) over one body.
+//   has_inc: GJK rule (GJK/cpu/openGJK.c:615-639): the incumbent support
+//            point keeps its place unless some vertex is strictly better; then
+//            the lowest index attaining the maximum wins.
+//   !has_inc: EPA rule (GJK/cpu/EPA.c:310-335): running max starts at -1e10,
+//            lowest index attaining the maximum wins; -1 if none beats -1e10.
+// The result is group-uniform.
+template <int LPP, int STRIDE>
+OGJK_DEV int support(const Body<float, STRIDE>& b, D3 dir, bool has_inc, int inc_idx, D3 inc_pt,
+                     int lane) {
+  // Scale the direction by an exact power of two so its max-norm is in
+  // [0.5, 1): the FP32 copy then carries a relative error <= 2^-24 per
+  // component whatever the magnitude of dir (tiny GJK directions included).
+  const double mx = fmax(fabs(dir.x), fmax(fabs(dir.y), fabs(dir.z)));
+  int e = 0;
+  (void)frexp(mx, &e);
+  const double sc = ldexp(1.0, -e);
+  const float dx = (float)(dir.x * sc), dy = (float)(dir.y * sc), dz = (float)(dir.z * sc);
+
+  float best = -INFINITY, second = -INFINITY;
+  int bi = -1;
+  for (int i = lane; i < b.n; i += LPP) {
+    const float d = fmaf(b.rx(i), dx, fmaf(b.ry(i), dy, b.rz(i) * dz));
+    second = fmaxf(second, fminf(best, d));
+    if (d > best) {
+      best = d;
+      bi = i;
+    }
+  }
+  const float M = group_max_f<LPP>(best);
+  // |D32 - sc*D64| <= 16u * r1 (u = 2^-24): candidates lie within 2x that.
+  const float T = M - 0x1.0p-19f * b.r1;
+  const bool multi = group_any<LPP>(second >= T);
+
+  double bv = -INFINITY;
+  int bidx = 0x7fffffff;
+  if (!multi) {
+    if (bi >= 0 && best >= T) {
+      bv = dot(b.d(bi), dir);
+      bidx = bi;
+    }
+  } else {
+    for (int i = lane; i < b.n; i += LPP) {
+      const float d = fmaf(b.rx(i), dx, fmaf(b.ry(i), dy, b.rz(i) * dz));
+      if (d >= T) {
+        const double v = dot(b.d(i), dir);
+        if (v > bv) {  // ascending i within the lane: keeps the lowest index
+          bv = v;
+          bidx = i;
+        }
+      }
+    }
+  }
+  if (!(bv == bv)) bv = -INFINITY;
+  group_argmax_d<LPP>(bv, bidx);
+  if (has_inc) return (bv > dot(inc_pt, dir)) ? bidx : inc_idx;
+  return (bv > -1e10) ? bidx : -1;
+}
+
+// Plain FP64 support over double storage: the same selection rules, computed
+// directly in FP64 (the comparison baseline; no FP32 pass).
+template <int LPP, int STRIDE>
+OGJK_DEV int support(const Body<double, STRIDE>& b, D3 dir, bool has_inc, int inc_idx, D3 inc_pt,
+                     int lane) {
+  double bv = -INFINITY;
+  int bidx = 0x7fffffff;
+  for (int i = lane; i < b.n; i += LPP) {
+    const double v = dot(b.d(i), dir);
+    if (v > bv) {
+      bv = v;
+      bidx = i;
+    }
+  }
+  if (!(bv == bv)) bv = -INFINITY;
+  group_argmax_d<LPP>(bv, bidx);
+  if (has_inc) return (bv > dot(inc_pt, dir)) ? bidx : inc_idx;
+  return (bv > -1e10) ? bidx : -1;
+}
+
+// --------------------------------------------------------------- simplex
+
+struct Simplex {
+  int n;
+  D3 p[4];
+  int ia[4], ib[4];
+};
+
+OGJK_DEV D3 sp(const Simplex& s, int k) {
+  return k == 0 ? s.p[0] : (k == 1 ? s.p[1] : (k == 2 ? s.p[2] : s.p[3]));
+}
+OGJK_DEV int sa(const Simplex& s, int k) {
+  return k == 0 ? s.ia[0] : (k == 1 ? s.ia[1] : (k == 2 ? s.ia[2] : s.ia[3]));
+}
+OGJK_DEV int sb(const Simplex& s, int k) {
+  return k == 0 ? s.ib[0] : (k == 1 ? s.ib[1] : (k == 2 ? s.ib[2] : s.ib[3]));
+}
+OGJK_DEV void sset(Simplex& s, int k, D3 p, int a, int b) {
+  if (k == 0) {
+    s.p[0] = p; s.ia[0] = a; s.ib[0] = b;
+  } else if (k == 1) {
+    s.p[1] = p; s.ia[1] = a; s.ib[1] = b;
+  } else if (k == 2) {
+    s.p[2] = p; s.ia[2] = a; s.ib[2] = b;
+  } else {
+    s.p[3] = p; s.ia[3] = a; s.ib[3] = b;
+  }
+}
+template <int DST, int SRC>
+OGJK_DEV void smove(Simplex& s) {
+  s.p[DST] = s.p[SRC];
+  s.ia[DST] = s.ia[SRC];
+  s.ib[DST] = s.ib[SRC];
+}
+
+struct Pt {  // a simplex vertex with its pair of body indices
+  D3 p;
+  int a, b;
+};
+OGJK_DEV Pt spt(const Simplex& s, int k) { return Pt{sp(s, k), sa(s, k), sb(s, k)}; }
+
+// select_1ik / 1ij / 1jk (GJK/cpu/openGJK.c:53-78): {newest, a, b}
+OGJK_DEV void keep3(Simplex& s, const Pt& a, const Pt& b) {
+  s.n = 3;
+  smove<2, 3>(s);
+  s.p[1] = a.p; s.ia[1] = a.a; s.ib[1] = a.b;
+  s.p[0] = b.p; s.ia[0] = b.a; s.ib[0] = b.b;
+}
+// select_1i / 1j / 1k (GJK/cpu/openGJK.c:80-99): {newest, a}
+OGJK_DEV void keep2(Simplex& s, const Pt& a) {
+  s.n = 2;
+  smove<1, 3>(s);
+  s.p[0] = a.p; s.ia[0] = a.a; s.ib[0] = a.b;
+}
+
+// ------------------------------------------------------- predicates (FP64)
+
+// determinant: GJK/cpu/openGJK.c:167-173
+OGJK_DEV double det3(D3 p, D3 q, D3 r) {
+  return p.x * ((q.y * r.z) - (r.y * q.z)) - p.y * (q.x * r.z - r.x * q.z) +
+         p.z * (q.x * r.y - r.x * q.y);
+}
+// projectOnLine: GJK/cpu/openGJK.c:183-196
+OGJK_DEV D3 proj_line(D3 p, D3 q) {
+  const D3 pq = sub(p, q);
+  const double t = dot(p, pq) / dot(pq, pq);
+  return D3{p.x - pq.x * t, p.y - pq.y * t, p.z - pq.z * t};
+}
+// projectOnPlane: GJK/cpu/openGJK.c:198-217
+OGJK_DEV D3 proj_plane(D3 p, D3 q, D3 r) {
+  const D3 n = cross(sub(p, q), sub(p, r));
+  const double t = dot(n, p) / dot(n, n);
+  return D3{n.x * t, n.y * t, n.z * t};
+}
+// hff1: GJK/cpu/openGJK.c:219-230 (sum starts at 0, as in the reference)
+OGJK_DEV int hff1(D3 p, D3 q) {
+  double t = 0;
+  t += (p.x * p.x - p.x * q.x);
+  t += (p.y * p.y - p.y * q.y);
+  t += (p.z * p.z - p.z * q.z);
+  return t > 0;
+}
+// hff2: GJK/cpu/openGJK.c:232-248
+OGJK_DEV int hff2(D3 p, D3 q, D3 r) {
+  const D3 pq = sub(q, p);
+  const D3 nt = cross(pq, sub(r, p));
+  const D3 n = cross(pq, nt);
+  return dot(p, n) < 0;
+}
+// hff3: GJK/cpu/openGJK.c:250-255
+OGJK_DEV int hff3(D3 p, D3 q, D3 r) { return dot(p, cross(q, r)) <= 0; }
+
+// ------------------------------------------------------- S1D / S2D / S3D
+
+// S1D: GJK/cpu/openGJK.c:257-268
+OGJK_DEV void S1D(Simplex& s, D3& v) {
+  if (hff1(s.p[1], s.p[0])) {
+    v = proj_line(s.p[1], s.p[0]);
+  } else {
+    v = s.p[1];
+    s.n = 1;
+    smove<0, 1>(s);
+  }
+}
+
+// S2D: GJK/cpu/openGJK.c:270-313
+OGJK_DEV void S2D(Simplex& s, D3& v) {
+  const D3 s1 = s.p[2], s2 = s.p[1], s3 = s.p[0];
+  const int h12 = hff1(s1, s2);
+  const int h13 = hff1(s1, s3);
+  if (h12) {
+    if (!hff2(s1, s2, s3)) {
+      if (h13) {
+        if (!hff2(s1, s3, s2)) {
+          v = proj_plane(s1, s2, s3);
+        } else {
+          v = proj_line(s1, s3);
+          s.n = 2;
+          smove<1, 2>(s);
+        }
+      } else {
+        v = proj_plane(s1, s2, s3);
+      }
+    } else {
+      v = proj_line(s1, s2);
+      s.n = 2;
+      smove<0, 2>(s);
+    }
+  } else if (h13) {
+    if (!hff2(s1, s3, s2)) {
+      v = proj_plane(s1, s2, s3);
+    } else {
+      v = proj_line(s1, s3);
+      s.n = 2;
+      smove<1, 2>(s);
+    }
+  } else {
+    v = s.p[2];
+    s.n = 1;
+    smove<0, 2>(s);
+  }
+}
+
+// S3D: GJK/cpu/openGJK.c:315-613.  (k, i, j) name the three older vertices
+// with k opposite the face under test, as in the reference.
+OGJK_DEV void S3D(Simplex& s, D3& v) {
+  const Pt P1{s.p[3], s.ia[3], s.ib[3]};
+  const D3 s1 = P1.p, s2 = s.p[2], s3 = s.p[1], s4 = s.p[0];
+  const Pt P2{s2, s.ia[2], s.ib[2]};
+  const Pt P3{s3, s.ia[1], s.ib[1]};
+  const D3 s1s2 = sub(s2, s1), s1s3 = sub(s3, s1), s1s4 = sub(s4, s1);
+
+  const int h1_2 = hff1(s1, s2);
+  const int h1_1 = hff1(s1, s3);
+  const int h1_0 = hff1(s1, s4);
+  const int line3 = h1_1, line4 = h1_0;
+  const int dot_total = h1_2 + line3 + line4;
+
+  if (dot_total == 0) {
+    v = s1;
+    s.n = 1;
+    s.p[0] = s1; s.ia[0] = P1.a; s.ib[0] = P1.b;
+    return;
+  }
+
+  const double det134 = det3(s1s3, s1s4, s1s2);
+  const int sss = (det134 <= 0);
+  int p2 = hff3(s1, s3, s4) - sss;
+  p2 = p2 * p2;
+  int p3 = hff3(s1, s4, s2) - sss;
+  p3 = p3 * p3;
+  int p4 = hff3(s1, s2, s3) - sss;
+  p4 = p4 * p4;
+  const int planes = p2 + p3 + p4;
+
+  if (planes == 3) {
+    v = D3{0.0, 0.0, 0.0};
+    s.n = 4;
+    return;
+  }
+  if (planes == 2) {
+    s.n = 3;
+    if (!p2) {
+      smove<2, 3>(s);
+    } else if (!p3) {
+      s.p[1] = P2.p; s.ia[1] = P2.a; s.ib[1] = P2.b;
+      smove<2, 3>(s);
+    } else if (!p4) {
+      s.p[0] = P3.p; s.ia[0] = P3.a; s.ib[0] = P3.b;
+      s.p[1] = P2.p; s.ia[1] = P2.a; s.ib[1] = P2.b;
+      smove<2, 3>(s);
+    }
+    S2D(s, v);
+    return;
+  }
+  if (planes == 1) {
+    s.n = 3;
+    int k, i, j;
+    if (p2) {
+      k = 2; i = 1; j = 0;
+    } else if (p3) {
+      k = 1; i = 0; j = 2;
+    } else {
+      k = 0; i = 2; j = 1;
+    }
+    const Pt si = spt(s, i), sj = spt(s, j), sk = spt(s, k);
+    const int hk = (k == 2) ? h1_2 : (k == 1 ? h1_1 : h1_0);
+    const int hi = (i == 2) ? h1_2 : (i == 1 ? h1_1 : h1_0);
+    const int hj = (j == 2) ? h1_2 : (j == 1 ? h1_1 : h1_0);
+    if (dot_total == 1) {
+      if (hk) {
+        if (!hff2(s1, sk.p, si.p)) {
+          keep3(s, si, sk);
+          v = proj_plane(s1, si.p, sk.p);
+        } else if (!hff2(s1, sk.p, sj.p)) {
+          keep3(s, sj, sk);
+          v = proj_plane(s1, sj.p, sk.p);
+        } else {
+          keep2(s, sk);
+          v = proj_line(s1, sk.p);
+        }
+      } else if (hi) {
+        if (!hff2(s1, si.p, sk.p)) {
+          keep3(s, si, sk);
+          v = proj_plane(s1, si.p, sk.p);
+        } else {
+          keep2(s, si);
+          v = proj_line(s1, si.p);
+        }
+      } else {
+        if (!hff2(s1, sj.p, sk.p)) {
+          keep3(s, sj, sk);
+          v = proj_plane(s1, sj.p, sk.p);
+        } else {
+          keep2(s, sj);
+          v = proj_line(s1, sj.p);
+        }
+      }
+    } else if (dot_total == 2) {
+      if (hi) {
+        if (!hff2(s1, sk.p, si.p)) {
+          if (!hff2(s1, si.p, sk.p)) {
+            keep3(s, si, sk);
+            v = proj_plane(s1, si.p, sk.p);
+          } else {
+            keep2(s, sk);
+            v = proj_line(s1, sk.p);
+          }
+        } else {
+          if (!hff2(s1, sk.p, sj.p)) {
+            keep3(s, sj, sk);
+            v = proj_plane(s1, sj.p, sk.p);
+          } else {
+            keep2(s, sk);
+            v = proj_line(s1, sk.p);
+          }
+        }
+      } else if (hj) {
+        if (!hff2(s1, sk.p, sj.p)) {
+          if (!hff2(s1, sj.p, sk.p)) {
+            keep3(s, sj, sk);
+            v = proj_plane(s1, sj.p, sk.p);
+          } else {
+            keep2(s, sj);
+            v = proj_line(s1, sj.p);
+          }
+        } else {
+          if (!hff2(s1, sk.p, si.p)) {
+            keep3(s, si, sk);
+            v = proj_plane(s1, si.p, sk.p);
+          } else {
+            keep2(s, sk);
+            v = proj_line(s1, sk.p);
+          }
+        }
+      }
+      // else: no region; v and the 3-vertex prefix stay (openGJK.c:494-496)
+    } else {  // dot_total == 3
+      const int h2ik = hff2(s1, si.p, sk.p);
+      const int h2jk = hff2(s1, sj.p, sk.p);
+      const int h2ki = hff2(s1, sk.p, si.p);
+      const int h2kj = hff2(s1, sk.p, sj.p);
+      if (h2ki == 1 && h2kj == 1) {
+        keep2(s, sk);
+        v = proj_line(s1, sk.p);
+      } else if (h2ki) {
+        if (h2jk) {
+          keep2(s, sj);
+          v = proj_line(s1, sj.p);
+        } else {
+          keep3(s, sj, sk);
+          v = proj_plane(s1, sk.p, sj.p);
+        }
+      } else {
+        if (h2ik) {
+          keep2(s, si);
+          v = proj_line(s1, si.p);
+        } else {
+          keep3(s, si, sk);
+          v = proj_plane(s1, sk.p, si.p);
+        }
+      }
+    }
+    return;
+  }
+  // planes == 0: origin outside all three faces (openGJK.c:538-608)
+  if (dot_total == 1) {
+    int k, i, j;
+    if (line3) {
+      k = 2; i = 1; j = 0;
+    } else if (line4) {
+      k = 1; i = 0; j = 2;
+    } else {
+      k = 0; i = 2; j = 1;
+    }
+    const Pt si = spt(s, i), sj = spt(s, j), sk = spt(s, k);
+    if (!hff2(s1, si.p, sj.p)) {
+      keep3(s, si, sj);
+      v = proj_plane(s1, si.p, sj.p);
+    } else if (!hff2(s1, si.p, sk.p)) {
+      keep3(s, si, sk);
+      v = proj_plane(s1, si.p, sk.p);
+    } else {
+      keep2(s, si);
+      v = proj_line(s1, si.p);
+    }
+  } else if (dot_total == 2) {
+    s.n = 3;
+    int k, i, j;
+    if (!line3) {
+      k = 2; i = 1; j = 0;
+    } else if (!line4) {
+      k = 1; i = 0; j = 2;
+    } else {
+      k = 0; i = 2; j = 1;
+    }
+    const Pt si = spt(s, i), sj = spt(s, j), sk = spt(s, k);
+    if (!hff2(s1, sj.p, sk.p)) {
+      if (!hff2(s1, sk.p, sj.p)) {
+        keep3(s, sj, sk);
+        v = proj_plane(s1, sj.p, sk.p);
+      } else if (!hff2(s1, sk.p, si.p)) {
+        keep3(s, si, sk);
+        v = proj_plane(s1, sk.p, si.p);
+      } else {
+        keep2(s, sk);
+        v = proj_line(s1, sk.p);
+      }
+    } else if (!hff2(s1, sj.p, si.p)) {
+      keep3(s, si, sj);
+      v = proj_plane(s1, si.p, sj.p);
+    } else {
+      keep2(s, sj);
+      v = proj_line(s1, sj.p);
+    }
+  }
+  // dot_total == 3 with planes == 0: nothing (the reference has no branch)
+}
+
+// subalgorithm: GJK/cpu/openGJK.c:641-656
+OGJK_DEV void subalgorithm(Simplex& s, D3& v) {
+  if (s.n == 4)
+    S3D(s, v);
+  else if (s.n == 3)
+    S2D(s, v);
+  else if (s.n == 2)
+    S1D(s, v);
+}
+
+// ------------------------------------------------------- witnesses W*D
+
+struct Wit {
+  D3 w0, w1;
+};
+
+template <class T, int STRIDE>
+OGJK_DEV void W0D(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, const Simplex& s, Wit& w) {
+  w.w0 = A.d(s.ia[0]);
+  w.w1 = B.d(s.ib[0]);
+}
+
+// Blend of original vertices: sum_k (coord_k * a_k), left to right.
+template <class T, int STRIDE>
+OGJK_DEV double blend_c(const Body<T, STRIDE>& b, const int* idx, const double* a, int m, int c) {
+  double acc = 0.0;
+  for (int k = 0; k < m; k++) {
+    const int i = idx[k];
+    const double x = (double)(c == 0 ? b.rx(i) : (c == 1 ? b.ry(i) : b.rz(i)));
+    acc = (k == 0) ? x * a[k] : acc + x * a[k];
+  }
+  return acc;
+}
+
+// W1D: GJK/cpu/openGJK.c:668-700 (det == 0 falls through, as the reference)
+template <class T, int STRIDE>
+OGJK_DEV void W1D(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, const Simplex& s, Wit& w) {
+  const D3 p = s.p[0], q = s.p[1];
+  const D3 pq = sub(q, p), po = neg(p);
+  const double det = dot(pq, pq);
+  if (det == 0.0) W0D(A, B, s, w);
+  const double a1 = dot(pq, po) / det;
+  const double a[2] = {1.0 - a1, a1};
+  const int ia[2] = {s.ia[0], s.ia[1]}, ib[2] = {s.ib[0], s.ib[1]};
+  w.w0 = D3{blend_c(A, ia, a, 2, 0), blend_c(A, ia, a, 2, 1), blend_c(A, ia, a, 2, 2)};
+  w.w1 = D3{blend_c(B, ib, a, 2, 0), blend_c(B, ib, a, 2, 1), blend_c(B, ib, a, 2, 2)};
+}
+
+// W2D: GJK/cpu/openGJK.c:702-795
+template <class T, int STRIDE>
+OGJK_DEV void W2D(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, Simplex& s, Wit& w) {
+  const D3 p = s.p[0], q = s.p[1], r = s.p[2];
+  const D3 pq = sub(q, p), pr = sub(r, p), po = neg(p);
+  const double T00 = dot(pq, pq), T01 = dot(pq, pr), T11 = dot(pr, pr);
+  const double det = T00 * T11 - T01 * T01;
+  if (det == 0.0) W1D(A, B, s, w);
+  const double b0 = dot(pq, po), b1 = dot(pr, po);
+  const double I00 = T11 / det, I01 = -T01 / det, I11 = T00 / det;
+  const double a1 = I00 * b0 + I01 * b1;
+  const double a2 = I01 * b0 + I11 * b1;
+  const double a0 = 1.0 - a1 - a2;
+  if (a0 < kEps64) {
+    s.n = 2;
+    smove<0, 2>(s);
+    W1D(A, B, s, w);
+  } else if (a1 < kEps64) {
+    s.n = 2;
+    smove<1, 2>(s);
+    W1D(A, B, s, w);
+  } else if (a2 < kEps64) {
+    s.n = 2;
+    W1D(A, B, s, w);
+  }
+  const double a[3] = {a0, a1, a2};
+  const int ia[3] = {s.ia[0], s.ia[1], s.ia[2]}, ib[3] = {s.ib[0], s.ib[1], s.ib[2]};
+  w.w0 = D3{blend_c(A, ia, a, 3, 0), blend_c(A, ia, a, 3, 1), blend_c(A, ia, a, 3, 2)};
+  w.w1 = D3{blend_c(B, ib, a, 3, 0), blend_c(B, ib, a, 3, 1), blend_c(B, ib, a, 3, 2)};
+}
+
+// W3D: GJK/cpu/openGJK.c:797-929
+template <class T, int STRIDE>
+OGJK_DEV void W3D(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, Simplex& s, Wit& w) {
+  const D3 p = s.p[0], q = s.p[1], r = s.p[2], t = s.p[3];
+  const D3 pq = sub(q, p), pr = sub(r, p), ps = sub(t, p), po = neg(p);
+  const double T00 = dot(pq, pq), T01 = dot(pq, pr), T02 = dot(pq, ps);
+  const double T11 = dot(pr, pr), T12 = dot(pr, ps), T22 = dot(ps, ps);
+  const double det00 = T11 * T22 - T12 * T12;
+  const double det01 = T01 * T22 - T02 * T12;
+  const double det02 = T01 * T12 - T02 * T11;
+  const double det = T00 * det00 - T01 * det01 + T02 * det02;
+  if (det == 0.0) W2D(A, B, s, w);
+  const double b0 = dot(pq, po), b1 = dot(pr, po), b2 = dot(ps, po);
+  const double det11 = T00 * T22 - T02 * T02;
+  const double det12 = T00 * T12 - T01 * T02;
+  const double det22 = T00 * T11 - T01 * T01;
+  const double I00 = det00 / det, I01 = -det01 / det, I02 = det02 / det;
+  const double I11 = det11 / det, I12 = -det12 / det, I22 = det22 / det;
+  const double a1 = I00 * b0 + I01 * b1 + I02 * b2;
+  const double a2 = I01 * b0 + I11 * b1 + I12 * b2;
+  const double a3 = I02 * b0 + I12 * b1 + I22 * b2;
+  const double a0 = 1.0 - a1 - a2 - a3;
+  if (a0 < kEps64) {
+    s.n = 3;
+    smove<0, 3>(s);
+    W2D(A, B, s, w);
+  } else if (a1 < kEps64) {
+    s.n = 3;
+    smove<1, 3>(s);
+    W2D(A, B, s, w);
+  } else if (a2 < kEps64) {
+    s.n = 3;
+    smove<2, 3>(s);
+    W2D(A, B, s, w);
+  } else if (a3 < kEps64) {
+    s.n = 3;
+    W2D(A, B, s, w);
+  }
+  const double a[4] = {a0, a1, a2, a3};
+  const int ia[4] = {s.ia[0], s.ia[1], s.ia[2], s.ia[3]};
+  const int ib[4] = {s.ib[0], s.ib[1], s.ib[2], s.ib[3]};
+  w.w0 = D3{blend_c(A, ia, a, 4, 0), blend_c(A, ia, a, 4, 1), blend_c(A, ia, a, 4, 2)};
+  w.w1 = D3{blend_c(B, ib, a, 4, 0), blend_c(B, ib, a, 4, 1), blend_c(B, ib, a, 4, 2)};
+}
+
+// compute_witnesses: GJK/cpu/openGJK.c:931-950
+template <class T, int STRIDE>
+OGJK_DEV void witnesses(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, Simplex& s, Wit& w) {
+  if (s.n == 4)
+    W3D(A, B, s, w);
+  else if (s.n == 3)
+    W2D(A, B, s, w);
+  else if (s.n == 2)
+    W1D(A, B, s, w);
+  else if (s.n == 1)
+    W0D(A, B, s, w);
+}
+
+// ------------------------------------------------------------- GJK driver
+
+// compute_minimum_distance: GJK/cpu/openGJK.c:952-1056.  Returns ||v||.
+template <int LPP, class T, int STRIDE>
+OGJK_DEV double gjk(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, Simplex& s, Wit& w, int lane) {
+  const double eps_rel = kEps64 * 1e4;  // (gkFloat)DBL_EPSILON * 1e4f
+  const double eps_tot = kEps64 * 1e2;
+  const double eps_rel2 = eps_rel * eps_rel;
+
+  D3 pa = A.d(0), pb = B.d(0);
+  int ia = 0, ib = 0;
+  D3 v = sub(pa, pb);
+  s.n = 1;
+  s.p[0] = v;
+  s.ia[0] = 0;
+  s.ib[0] = 0;
+  double norm2Wmax = 0.0;
+  unsigned k = 0;
+  do {
+    k++;
+    const int na = support<LPP>(A, neg(v), true, ia, pa, lane);
+    if (na != ia) {
+      ia = na;
+      pa = A.d(ia);
+    }
+    const int nb = support<LPP>(B, v, true, ib, pb, lane);
+    if (nb != ib) {
+      ib = nb;
+      pb = B.d(ib);
+    }
+    const D3 wv = sub(pa, pb);
+    const double gap = norm2(v) - dot(v, wv);
+    if (gap <= eps_rel * norm2(v) || gap < eps_tot) break;
+    if (norm2(v) < eps_rel2) break;
+
+    sset(s, s.n, wv, ia, ib);
+    s.n++;
+    subalgorithm(s, v);
+
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) {
+      if (jj < s.n) {
+        const double tn = norm2(s.p[jj]);
+        if (tn > norm2Wmax) norm2Wmax = tn;
+      }
+    }
+    if (norm2(v) <= eps_tot * eps_tot * norm2Wmax) break;
+  } while (s.n != 4 && k != 25u);
+
+  witnesses(A, B, s, w);
+  return sqrt(norm2(v));
+}
+
+}  // namespace ogjk

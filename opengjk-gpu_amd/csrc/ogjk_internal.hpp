@@ -1,0 +1,52 @@
+// Internal launch interface between the C ABI (ogjk_api.hip) and the kernels
+// (ogjk_kernels.hip).  Host-side only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "opengjk/common.h"
+
+namespace ogjk {
+
+// Where a pair's two bodies come from.
+struct SceneArgs {       // SoA float scene (or double for the plain path)
+  const void* x;
+  const void* y;
+  const void* z;
+  const int64_t* off;
+  const int32_t* cnt;
+  const float* r1;       // may be null for the double scene
+  int32_t n_bodies;
+  const int32_t* pairs;  // 2 per pair, or null: pair i = (2i, 2i+1)
+};
+
+struct PolyArgs {                // gkPolytope descriptors, AoS coordinates
+  const gkPolytope* a;           // pair i: (a[i], b[i]) when pairs is null
+  const gkPolytope* b;
+  const gkCollisionPair* pairs;  // else (a[pairs[i].idx1], a[pairs[i].idx2])
+  int32_t n_pool;
+};
+
+struct OutArgs {
+  gkSimplex* simplices;  // GJK: out; EPA: in/out
+  float* distances;      // GJK: out; EPA: in/out
+  float* normals;        // EPA: out (3 per pair), may be null
+  double* f64;           // optional FP64 verification record, may be null
+};
+
+// Doubles per pair in the optional FP64 verification record.
+constexpr int OGJK_F64_REC = 24;
+
+// Lanes per pair used by the launchers.
+constexpr int kLanesPerPair = 64;
+
+hipError_t launch_scene(int mode, const SceneArgs& s, const OutArgs& o, int n, hipStream_t st);
+hipError_t launch_scene_f64(int mode, const SceneArgs& s, const OutArgs& o, int n, hipStream_t st);
+hipError_t launch_poly(int mode, const PolyArgs& p, const OutArgs& o, int n, hipStream_t st);
+hipError_t launch_pack_f64(const float* aos, int64_t n_verts, double* x, double* y, double* z,
+                           hipStream_t st);
+hipError_t launch_pack(const float* aos, const int64_t* off, const int32_t* cnt, int n_bodies,
+                       int64_t n_verts, float* x, float* y, float* z, float* r1, hipStream_t st);
+
+}  // namespace ogjk

@@ -1,0 +1,122 @@
+// opengjk-amd device math: FP64 vector helpers and wave64 group collectives.
+//
+// Everything here is compiled with FMA contraction off (the build passes
+// -ffp-contract=off and the pragma below), so each a*b+c rounds twice, exactly
+// like the CPU checker.  FMAs are only ever written explicitly.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace ogjk {
+
+#define OGJK_DEV __device__ __forceinline__
+
+// DBL_EPSILON / FLT_EPSILON as the reference uses them (GJK/common.h:48,55).
+constexpr double kEps64 = 2.220446049250313080847e-16;
+constexpr float kEps32 = 1.1920928955078125e-07f;
+
+struct D3 {
+  double x, y, z;
+};
+
+OGJK_DEV D3 d3(double x, double y, double z) { return D3{x, y, z}; }
+OGJK_DEV D3 sub(D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+OGJK_DEV D3 neg(D3 a) { return D3{-a.x, -a.y, -a.z}; }
+// (a0*b0 + a1*b1) + a2*b2, the association of the reference's dotProduct macro.
+OGJK_DEV double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+OGJK_DEV double norm2(D3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+OGJK_DEV D3 cross(D3 a, D3 b) {
+  return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+OGJK_DEV D3 sel(bool c, D3 a, D3 b) { return c ? a : b; }
+
+// ------------------------------------------------------------ collectives
+// A "group" is LPP consecutive lanes of one wave64 (LPP in {8,16,32,64}).
+// All lanes of a group are kept convergent around every collective.
+
+OGJK_DEV int lane_id() { return __lane_id(); }
+
+template <int LPP>
+OGJK_DEV uint64_t group_ballot(bool p) {
+  const uint64_t b = __ballot(p);
+  if constexpr (LPP == 64) {
+    return b;
+  } else {
+    const int base = lane_id() & ~(LPP - 1);
+    return (b >> base) & ((1ull << LPP) - 1);
+  }
+}
+
+template <int LPP>
+OGJK_DEV bool group_any(bool p) {
+  return group_ballot<LPP>(p) != 0;
+}
+
+// Number of set predicates in lower lanes of the group.
+template <int LPP>
+OGJK_DEV int group_prefix(uint64_t gb) {
+  const int l = lane_id() & (LPP - 1);
+  return __popcll(gb & ((1ull << l) - 1ull));
+}
+
+template <int LPP>
+OGJK_DEV float shfl_xor_f(float v, int m) {
+  return __shfl_xor(v, m, LPP);
+}
+template <int LPP>
+OGJK_DEV int shfl_xor_i(int v, int m) {
+  return __shfl_xor(v, m, LPP);
+}
+template <int LPP>
+OGJK_DEV double shfl_xor_d(double v, int m) {
+  return __shfl_xor(v, m, LPP);
+}
+template <int LPP>
+OGJK_DEV double shfl_d(double v, int src) {
+  return __shfl(v, src, LPP);
+}
+template <int LPP>
+OGJK_DEV int shfl_i(int v, int src) {
+  return __shfl(v, src, LPP);
+}
+
+template <int LPP>
+OGJK_DEV float group_max_f(float v) {
+#pragma unroll
+  for (int m = LPP / 2; m >= 1; m >>= 1) v = fmaxf(v, shfl_xor_f<LPP>(v, m));
+  return v;
+}
+
+// Max of (value, index) with the lowest index winning ties.  NaN values must
+// already be mapped to -inf by the caller, so the result is group-uniform.
+template <int LPP>
+OGJK_DEV void group_argmax_d(double& v, int& i) {
+#pragma unroll
+  for (int m = LPP / 2; m >= 1; m >>= 1) {
+    const double ov = shfl_xor_d<LPP>(v, m);
+    const int oi = shfl_xor_i<LPP>(i, m);
+    if (ov > v || (ov == v && oi < i)) {
+      v = ov;
+      i = oi;
+    }
+  }
+}
+
+// Min of (value, index), lowest index on ties; NaN must be mapped to +inf.
+template <int LPP>
+OGJK_DEV void group_argmin_d(double& v, int& i) {
+#pragma unroll
+  for (int m = LPP / 2; m >= 1; m >>= 1) {
+    const double ov = shfl_xor_d<LPP>(v, m);
+    const int oi = shfl_xor_i<LPP>(i, m);
+    if (ov < v || (ov == v && oi < i)) {
+      v = ov;
+      i = oi;
+    }
+  }
+}
+
+}  // namespace ogjk

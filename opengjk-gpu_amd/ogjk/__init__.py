@@ -1,0 +1,218 @@
+"""Python host bindings for libogjk.so (the C ABI in include/opengjk/ogjk.h).
+
+This is plumbing for tests and the benchmark: the library itself is C++/HIP.
+Device buffers for the resident-scene entry points come from torch (ROCm);
+the host entry points take numpy arrays.  Every call raises OgjkError on a
+non-zero status; there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libogjk.so")
+
+GJK, EPA, GJK_EPA = 1, 2, 3
+F64_RECORD = 24
+
+SIMPLEX_DTYPE = np.dtype({
+    "names": ["nvrtx", "vrtx", "vrtx_idx", "witnesses"],
+    "formats": [np.int32, (np.float32, (4, 3)), (np.int32, (4, 2)), (np.float32, (2, 3))],
+    "offsets": [0, 4, 52, 84],
+    "itemsize": 108,
+})
+POLYTOPE_DTYPE = np.dtype({
+    "names": ["numpoints", "s", "s_idx", "coord"],
+    "formats": [np.int32, (np.float32, 3), np.int32, np.uint64],
+    "offsets": [0, 4, 16, 24],
+    "itemsize": 32,
+})
+PAIR_DTYPE = np.dtype([("idx1", np.int32), ("idx2", np.int32)])
+
+# Every symbol include/opengjk/ogjk.h declares (checked by the ABI test).
+C_ABI_SYMBOLS = [
+    "ogjk_version", "ogjk_last_error", "ogjk_device_count", "ogjk_sync", "ogjk_pack_scene",
+    "ogjk_collide_scene", "ogjk_pack_scene_f64", "ogjk_collide_scene_f64",
+    "ogjk_collide_polytopes", "ogjk_collide_polytopes_indexed", "ogjk_collide_host",
+    "ogjk_collide_host_indexed",
+]
+
+
+class OgjkError(RuntimeError):
+    pass
+
+
+class _Scene(C.Structure):
+    _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("body_off", C.c_void_p),
+                ("body_cnt", C.c_void_p), ("body_r1", C.c_void_p), ("n_bodies", C.c_int32)]
+
+
+class _Scene64(C.Structure):
+    _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("body_off", C.c_void_p),
+                ("body_cnt", C.c_void_p), ("n_bodies", C.c_int32)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libogjk.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OgjkError(f"{LIB_PATH} missing: run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+        sig = {
+            "ogjk_version": (C.c_int, []),
+            "ogjk_last_error": (C.c_char_p, []),
+            "ogjk_device_count": (C.c_int, []),
+            "ogjk_sync": (C.c_int, [vp]),
+            "ogjk_pack_scene": (C.c_int, [vp, vp, vp, i32, i64, vp, vp, vp, vp, vp]),
+            "ogjk_pack_scene_f64": (C.c_int, [vp, i64, vp, vp, vp, vp]),
+            "ogjk_collide_scene": (C.c_int, [C.c_int, C.POINTER(_Scene), vp, i32, vp, vp, vp, vp, vp]),
+            "ogjk_collide_scene_f64": (C.c_int, [C.c_int, C.POINTER(_Scene64), vp, i32, vp, vp, vp, vp, vp]),
+            "ogjk_collide_polytopes": (C.c_int, [C.c_int, vp, vp, i32, vp, vp, vp, vp]),
+            "ogjk_collide_polytopes_indexed": (C.c_int, [C.c_int, vp, i32, vp, i32, vp, vp, vp, vp]),
+            "ogjk_collide_host": (C.c_int, [C.c_int, i32, vp, vp, vp, vp, vp]),
+            "ogjk_collide_host_indexed": (C.c_int, [C.c_int, i32, i32, vp, vp, vp, vp, vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        msg = lib().ogjk_last_error().decode(errors="replace")
+        raise OgjkError(f"ogjk status {status}: {msg}")
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def device_count() -> int:
+    return int(lib().ogjk_device_count())
+
+
+# ------------------------------------------------------------------ host API
+
+def polytopes(coords: np.ndarray, off, cnt, bodies) -> np.ndarray:
+    """gkPolytope records (FP32 ABI) pointing into an AoS float32 pool."""
+    assert coords.dtype == np.float32 and coords.flags.c_contiguous
+    base = coords.ctypes.data
+    bodies = np.asarray(bodies)
+    p = np.zeros(len(bodies), dtype=POLYTOPE_DTYPE)
+    p["numpoints"] = np.asarray(cnt)[bodies]
+    p["coord"] = base + 12 * np.asarray(off, dtype=np.int64)[bodies].astype(np.uint64)
+    return p
+
+
+def collide_host(mode: int, batch, simplices=None, distances=None):
+    """ogjk_collide_host over a generator Batch: pair i = (pa[i], pb[i])."""
+    n = batch.n_pairs
+    bd1 = polytopes(batch.coords, batch.off, batch.cnt, batch.pa)
+    bd2 = polytopes(batch.coords, batch.off, batch.cnt, batch.pb)
+    simp = np.zeros(n, SIMPLEX_DTYPE) if simplices is None else np.array(simplices, SIMPLEX_DTYPE)
+    dist = np.zeros(n, np.float32) if distances is None else np.array(distances, np.float32)
+    nrm = np.zeros((n, 3), np.float32)
+    _check(lib().ogjk_collide_host(mode, n, _ptr(bd1), _ptr(bd2), _ptr(simp), _ptr(dist), _ptr(nrm)))
+    return simp, dist, nrm
+
+
+def collide_host_indexed(mode: int, coords, off, cnt, pairs, simplices=None, distances=None):
+    pool = polytopes(coords, off, cnt, np.arange(len(cnt)))
+    pr = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+    n = len(pr)
+    simp = np.zeros(n, SIMPLEX_DTYPE) if simplices is None else np.array(simplices, SIMPLEX_DTYPE)
+    dist = np.zeros(n, np.float32) if distances is None else np.array(distances, np.float32)
+    nrm = np.zeros((n, 3), np.float32)
+    _check(lib().ogjk_collide_host_indexed(mode, len(cnt), n, _ptr(pool), _ptr(pr), _ptr(simp),
+                                           _ptr(dist), _ptr(nrm)))
+    return simp, dist, nrm
+
+
+# ------------------------------------------------------------ device scenes
+
+class DeviceScene:
+    """A batch resident in device memory as an SoA scene (torch tensors)."""
+
+    def __init__(self, batch, device="cuda", f64: bool = False, stream=None):
+        import torch
+        self.torch = torch
+        self.n_bodies = len(batch.cnt)
+        self.n_verts = int(batch.coords.shape[0])
+        dev = torch.device(device)
+        aos = torch.from_numpy(np.ascontiguousarray(batch.coords).reshape(-1)).to(dev)
+        self.off = torch.from_numpy(np.ascontiguousarray(batch.off, dtype=np.int64)).to(dev)
+        self.cnt = torch.from_numpy(np.ascontiguousarray(batch.cnt, dtype=np.int32)).to(dev)
+        nv = max(self.n_verts, 1)
+        self.x = torch.empty(nv, dtype=torch.float32, device=dev)
+        self.y = torch.empty(nv, dtype=torch.float32, device=dev)
+        self.z = torch.empty(nv, dtype=torch.float32, device=dev)
+        self.r1 = torch.empty(max(self.n_bodies, 1), dtype=torch.float32, device=dev)
+        st = self._stream(stream)
+        _check(lib().ogjk_pack_scene(aos.data_ptr(), self.off.data_ptr(), self.cnt.data_ptr(),
+                                     self.n_bodies, self.n_verts, self.x.data_ptr(),
+                                     self.y.data_ptr(), self.z.data_ptr(), self.r1.data_ptr(), st))
+        self.f64 = None
+        if f64:
+            self.x64 = torch.empty(nv, dtype=torch.float64, device=dev)
+            self.y64 = torch.empty(nv, dtype=torch.float64, device=dev)
+            self.z64 = torch.empty(nv, dtype=torch.float64, device=dev)
+            _check(lib().ogjk_pack_scene_f64(aos.data_ptr(), self.n_verts, self.x64.data_ptr(),
+                                             self.y64.data_ptr(), self.z64.data_ptr(), st))
+            self.f64 = _Scene64(self.x64.data_ptr(), self.y64.data_ptr(), self.z64.data_ptr(),
+                                self.off.data_ptr(), self.cnt.data_ptr(), self.n_bodies)
+        torch.cuda.synchronize()
+        del aos
+        self.scene = _Scene(self.x.data_ptr(), self.y.data_ptr(), self.z.data_ptr(),
+                            self.off.data_ptr(), self.cnt.data_ptr(), self.r1.data_ptr(),
+                            self.n_bodies)
+
+    def _stream(self, stream):
+        if stream is None:
+            return self.torch.cuda.current_stream().cuda_stream
+        return stream.cuda_stream if hasattr(stream, "cuda_stream") else stream
+
+    def outputs(self, n: int, f64: bool = False):
+        t = self.torch
+        dev = self.x.device
+        return {
+            "simplices": t.zeros((n, 27), dtype=t.int32, device=dev),  # 108 B records
+            "distances": t.zeros(n, dtype=t.float32, device=dev),
+            "normals": t.zeros((n, 3), dtype=t.float32, device=dev),
+            "f64": t.zeros((n, F64_RECORD), dtype=t.float64, device=dev) if f64 else None,
+        }
+
+    def collide(self, mode: int, n_pairs: int, pairs=None, out=None, plain_f64: bool = False,
+                f64_record: bool = False, stream=None):
+        """Launch on `stream` (default: torch's current stream); asynchronous."""
+        out = out or self.outputs(n_pairs, f64_record)
+        pp = None if pairs is None else pairs.data_ptr()
+        f = out["f64"].data_ptr() if out.get("f64") is not None else None
+        st = self._stream(stream)
+        if plain_f64:
+            if self.f64 is None:
+                raise OgjkError("scene built without f64=True")
+            _check(lib().ogjk_collide_scene_f64(mode, C.byref(self.f64), pp, n_pairs,
+                                                out["simplices"].data_ptr(),
+                                                out["distances"].data_ptr(),
+                                                out["normals"].data_ptr(), f, st))
+        else:
+            _check(lib().ogjk_collide_scene(mode, C.byref(self.scene), pp, n_pairs,
+                                            out["simplices"].data_ptr(), out["distances"].data_ptr(),
+                                            out["normals"].data_ptr(), f, st))
+        return out
+
+
+def simplices_from_device(t) -> np.ndarray:
+    """(n, 27) int32 device tensor of gkSimplex records -> numpy records."""
+    return np.ascontiguousarray(t.cpu().numpy()).view(SIMPLEX_DTYPE).reshape(-1)

@@ -1,0 +1,105 @@
+"""Hand-made and tie-heavy test inputs.
+
+Known-answer cases follow the reference's own tests:
+  * userP/userQ point sets (examples/usage/userP.dat, userQ.dat) with the
+    expected distance 3.653650 (README.md:111-115);
+  * the EPAUsage cube pair (examples/usage/EPAUsage.cpp:31-82), expected
+    depth 1.5, witnesses (1, .5, .707107) / (-.5, .5, .707107), normal
+    (1, -0, 0) (README.md:131-137);
+  * the EPATesting cube cases (examples/gpu/example.cu:394-580): cube vs
+    cube+1 (depth band 0.8-1.2), touching cube+2 (0 <= d < 0.01), cube+5
+    (distance 3, nvrtx < 4).
+"""
+from __future__ import annotations
+
+import math
+import os
+
+import numpy as np
+
+from ogjk import gen
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+CUBE = np.array([[x, y, z] for x in (-1, 1) for y in (-1, 1) for z in (-1, 1)], np.float32)
+
+
+def read_dat(name: str) -> np.ndarray:
+    """Point-set format of examples/main.cpp:31-77: count, then x y z rows."""
+    with open(os.path.join(GOLDEN, name)) as f:
+        tok = f.read().split()
+    n = int(tok[0])
+    return np.array(tok[1:1 + 3 * n], dtype=np.float32).reshape(n, 3)
+
+
+def epa_usage_cubes():
+    """Cube and a cube rotated 45 deg about x, y, z then moved +1 in x, built
+    in float as examples/usage/EPAUsage.cpp:23-82 does."""
+    f = np.float32
+    ang = f(f(45.0) * f(math.pi) / f(180.0))
+    ca, sa = f(math.cos(float(ang))), f(math.sin(float(ang)))
+    c2 = []
+    for x in (-1, 1):
+        for y in (-1, 1):
+            for z in (-1, 1):
+                px, py, pz = f(x), f(y), f(z)
+                ty = f(py * ca - pz * sa)
+                tz = f(py * sa + pz * ca)
+                py, pz = ty, tz
+                tx = f(px * ca + pz * sa)
+                tz = f(-px * sa + pz * ca)
+                px, pz = tx, tz
+                tx = f(px * ca - py * sa)
+                ty = f(px * sa + py * ca)
+                px, py = tx, ty
+                c2.append([px + f(1.0), py, pz])
+    return CUBE.copy(), np.array(c2, np.float32)
+
+
+def known_answer_batch():
+    P, Q = read_dat("userP.dat"), read_dat("userQ.dat")
+    c1, c2 = epa_usage_cubes()
+    A = [P, c1, CUBE, CUBE, CUBE]
+    B = [Q, c2, CUBE + np.float32([1, 0, 0]), CUBE + np.float32([2, 0, 0]), CUBE + np.float32([5, 0, 0])]
+    return gen.from_bodies(A, B)
+
+
+def degenerate_batch(n: int = 4000, seed: int = 5):
+    """Tie-heavy and degenerate pairs: lattice cubes, integer clouds, flat
+    sets, 1-3 point bodies, duplicated vertices, grid cubes, deep overlap,
+    scales from 1e-3 to 1e3."""
+    rng = np.random.default_rng(seed)
+    A, B = [], []
+    g = np.linspace(-1, 1, 5, dtype=np.float32)
+    grid = np.array([[x, y, z] for x in g for y in g for z in g if max(abs(x), abs(y), abs(z)) == 1],
+                    np.float32)
+    for i in range(n):
+        k = i % 8
+        if k == 0:
+            A.append(CUBE); B.append(CUBE + rng.integers(-3, 4, size=3).astype(np.float32))
+        elif k == 1:
+            n1, n2 = rng.integers(1, 40, size=2)
+            A.append(rng.integers(-2, 3, size=(n1, 3)).astype(np.float32))
+            B.append(rng.integers(-2, 3, size=(n2, 3)).astype(np.float32))
+        elif k == 2:
+            n1, n2 = rng.integers(3, 30, size=2)
+            a = rng.normal(size=(n1, 3)).astype(np.float32); a[:, 2] = 0
+            b = rng.normal(size=(n2, 3)).astype(np.float32); b[:, 2] = rng.integers(-1, 2)
+            A.append(a); B.append(b)
+        elif k == 3:
+            n1, n2 = rng.integers(1, 4, size=2)
+            A.append(rng.normal(size=(n1, 3)).astype(np.float32))
+            B.append(rng.normal(size=(n2, 3)).astype(np.float32) * np.float32(0.5))
+        elif k == 4:
+            a = rng.normal(size=(10, 3)).astype(np.float32); a = np.concatenate([a, a[::-1]])
+            A.append(a); B.append(a * np.float32(0.7) + rng.normal(size=3).astype(np.float32) * np.float32(0.5))
+        elif k == 5:
+            A.append(grid); B.append(grid * np.float32(0.5) + rng.integers(-4, 5, size=3).astype(np.float32) * np.float32(0.25))
+        elif k == 6:
+            A.append(rng.normal(size=(50, 3)).astype(np.float32))
+            B.append(rng.normal(size=(50, 3)).astype(np.float32) + np.float32(0.1))
+        else:
+            s = np.float32(10.0 ** rng.integers(-3, 4))
+            A.append(rng.normal(size=(20, 3)).astype(np.float32) * s)
+            B.append(rng.normal(size=(20, 3)).astype(np.float32) * s + s * rng.normal(size=3).astype(np.float32))
+    return gen.from_bodies(A, B)

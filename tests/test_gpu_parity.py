@@ -1,0 +1,124 @@
+"""GPU parity: the HIP path, called through the C ABI, against the FP64 oracle
+on the same seeded inputs.  Tolerances are defined in parity_utils."""
+import numpy as np
+import pytest
+
+import oracle
+import ogjk
+from ogjk import gen
+import cases
+from parity_utils import compare_f32, compare_f64, oracle_run
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    assert ogjk.device_count() >= 1
+    return torch
+
+
+def run_scene(torch, batch, mode, plain=False):
+    sc = ogjk.DeviceScene(batch, f64=plain)
+    out = sc.collide(mode, batch.n_pairs, f64_record=True, plain_f64=plain)
+    torch.cuda.synchronize()
+    rec = out["f64"].cpu().numpy()
+    simp = ogjk.simplices_from_device(out["simplices"])
+    return rec, simp, out["distances"].cpu().numpy(), out["normals"].cpu().numpy()
+
+
+def check(res, name, min_exact=1.0):
+    print(f"\n[{name}] {res}", flush=True)
+    assert res["within_tol"] == res["n"], f"{name}: out of tolerance at pairs {res['bad']}"
+    assert res["exact"] >= min_exact * res["n"], f"{name}: inexact pairs {res['inexact']}"
+
+
+WORKLOADS = {
+    "c0_sphere32_box2": dict(family=gen.SPHERE, n_pairs=1000, seed=0, nverts=32, box=2.0),
+    "c1_sphere64_box1.3": dict(family=gen.SPHERE, n_pairs=4096, seed=1, nverts=64, box=1.3),
+    "c2_ellipsoid_4_4096": dict(family=gen.ELLIPSOID, n_pairs=384, seed=2, box=1.5, nmin=4, nmax=4096),
+    "c3_noisy_sphere_2000": dict(family=gen.NOISY_SPHERE, n_pairs=64, seed=3, nverts=2000, box=1.4, sigma=0.01),
+}
+
+
+@pytest.mark.parametrize("name", list(WORKLOADS))
+@pytest.mark.parametrize("mode", [oracle.GJK, oracle.GJK_EPA])
+def test_workload_parity(torch_cuda, name, mode):
+    b = gen.make(**WORKLOADS[name])
+    ref = oracle_run(mode, b)
+    rec, simp, d32, n32 = run_scene(torch_cuda, b, mode)
+    check(compare_f64(rec, simp, ref, mode), f"{name} mode{mode}")
+    r32 = compare_f32(simp, d32, n32, ref, mode)
+    assert r32["exact"] == r32["n"], r32
+
+
+@pytest.mark.parametrize("mode", [oracle.GJK, oracle.GJK_EPA])
+def test_degenerate_parity(torch_cuda, mode):
+    b = cases.degenerate_batch(4000)
+    ref = oracle_run(mode, b)
+    rec, simp, d32, n32 = run_scene(torch_cuda, b, mode)
+    check(compare_f64(rec, simp, ref, mode), f"degenerate mode{mode}")
+
+
+def test_plain_f64_equals_mixed(torch_cuda):
+    b = gen.make(gen.SPHERE, 2048, seed=11, nverts=64, box=1.3)
+    rec_m, simp_m, _, _ = run_scene(torch_cuda, b, oracle.GJK_EPA)
+    rec_p, simp_p, _, _ = run_scene(torch_cuda, b, oracle.GJK_EPA, plain=True)
+    assert np.array_equal(rec_m.view(np.uint64), rec_p.view(np.uint64))
+    assert np.array_equal(simp_m.view(np.uint8), simp_p.view(np.uint8))
+
+
+def test_known_answers_dropin(torch_cuda):
+    b = cases.known_answer_batch()
+    simp, dist, nrm = ogjk.collide_host(oracle.GJK_EPA, b)
+    assert abs(dist[0] - 3.653650) < 1e-5                      # README.md:111-115
+    assert np.allclose(simp["witnesses"][0, 0], [1.025173, 1.490318, 0.255463], atol=1e-6)
+    assert np.allclose(simp["witnesses"][0, 1], [-1.025173, -1.490318, -0.255463], atol=1e-6)
+    assert abs(-dist[1] - 1.5) < 1e-6                           # README.md:131-137
+    assert np.allclose(simp["witnesses"][1, 0], [1.0, 0.5, 0.707107], atol=1e-6)
+    assert np.allclose(simp["witnesses"][1, 1], [-0.5, 0.5, 0.707107], atol=1e-6)
+    assert np.allclose(nrm[1], [1.0, 0.0, 0.0], atol=1e-6)
+    assert -1.2 < dist[2] < -0.8                                # example.cu:450
+    assert 0.0 <= dist[3] < 0.01 or dist[3] == 0.0             # example.cu:515
+    assert 2.9 < dist[4] < 3.1 and simp["nvrtx"][4] < 4         # example.cu:580
+    ref = oracle_run(oracle.GJK_EPA, b)
+    r = compare_f32(simp, dist, nrm, ref, oracle.GJK_EPA)
+    assert r["exact"] == r["n"], r
+
+
+def test_host_paths_agree(torch_cuda):
+    """API consistency in the manner of examples/main.cpp:378-644."""
+    b = gen.make(gen.SPHERE, 1500, seed=4, nverts=64, box=1.3)
+    ref = oracle_run(oracle.GJK_EPA, b)
+    s1, d1, n1 = ogjk.collide_host(oracle.GJK_EPA, b)
+    assert compare_f32(s1, d1, n1, ref, oracle.GJK_EPA)["exact"] == b.n_pairs
+    # GJK then EPA as two calls (computeDistances + computeEPA)
+    sg, dg, _ = ogjk.collide_host(oracle.GJK, b)
+    se, de, ne = ogjk.collide_host(oracle.EPA, b, simplices=sg, distances=dg)
+    r = compare_f32(se, de, ne, ref, oracle.GJK_EPA)
+    assert r["max_dist_rel"] <= 1e-5, r
+    wdiff = np.abs(se["witnesses"] - s1["witnesses"]).max()
+    assert wdiff <= 1e-4, wdiff
+    # indexed form over a shuffled pool
+    rng = np.random.default_rng(0)
+    perm = rng.permutation(len(b.cnt))
+    inv = np.argsort(perm)
+    coords = np.concatenate([b.body(int(j)) for j in perm]).astype(np.float32)
+    cnt = b.cnt[perm]
+    off = np.zeros(len(cnt), np.int64)
+    off[1:] = np.cumsum(cnt[:-1])
+    pairs = np.stack([inv[b.pa], inv[b.pb]], 1)
+    si, di, ni = ogjk.collide_host_indexed(oracle.GJK_EPA, np.ascontiguousarray(coords), off, cnt, pairs)
+    assert np.array_equal(di.view(np.uint32), d1.view(np.uint32))
+    assert np.array_equal(ni.view(np.uint32), n1.view(np.uint32))
+
+
+def test_invalid_pairs_flagged(torch_cuda):
+    b = gen.make(gen.SPHERE, 8, seed=9, nverts=16, box=1.0)
+    pairs = np.array([[0, 1], [2, 99], [-1, 3], [4, 5]], np.int32)
+    s, d, n = ogjk.collide_host_indexed(oracle.GJK_EPA, b.coords, b.off, b.cnt, pairs)
+    assert np.isnan(d[1]) and np.isnan(d[2]) and s["nvrtx"][1] == 0
+    assert np.isfinite(d[0]) and np.isfinite(d[3])
