@@ -76,6 +76,10 @@ const char* ogjk_last_error(void);
 /* Number of visible HIP devices (0 without a GPU; never fails). */
 int ogjk_device_count(void);
 ogjk_status ogjk_sync(void* stream);
+/* Lanes of a wave64 that work on one pair: 16, 32 or 64 (default 64).
+ * Process-wide setting read at launch time; not synchronised with launches
+ * running on other threads. */
+ogjk_status ogjk_set_lanes_per_pair(int lanes);
 
 /* ------------------------------------------------------------ scenes */
 

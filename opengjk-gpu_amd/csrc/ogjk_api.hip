@@ -161,6 +161,12 @@ int ogjk_device_count(void) {
   return n;
 }
 
+ogjk_status ogjk_set_lanes_per_pair(int lanes) {
+  if (ogjk::set_lanes_per_pair(lanes) != 0)
+    return fail(OGJK_ERR_INVALID, "lanes per pair must be 16, 32 or 64 (got %d)", lanes);
+  return OGJK_OK;
+}
+
 ogjk_status ogjk_sync(void* stream) {
   OGJK_HIP(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
   return OGJK_OK;

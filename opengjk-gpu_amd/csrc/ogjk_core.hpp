@@ -141,24 +141,43 @@ struct Simplex {
   int ia[4], ib[4];
 };
 
+// Runtime-indexed access to the simplex.  LLVM would turn a select between
+// two loads of the simplex into a load from a selected address, which forces
+// the whole simplex into scratch memory; reading every slot and passing the
+// values through an empty asm statement keeps the selects on registers.
+OGJK_DEV double keep_reg(double v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+OGJK_DEV int keep_reg(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+OGJK_DEV D3 keep_reg(D3 v) { return D3{keep_reg(v.x), keep_reg(v.y), keep_reg(v.z)}; }
+
 OGJK_DEV D3 sp(const Simplex& s, int k) {
-  return k == 0 ? s.p[0] : (k == 1 ? s.p[1] : (k == 2 ? s.p[2] : s.p[3]));
+  const D3 p0 = keep_reg(s.p[0]), p1 = keep_reg(s.p[1]), p2 = keep_reg(s.p[2]),
+           p3 = keep_reg(s.p[3]);
+  return sel(k == 0, p0, sel(k == 1, p1, sel(k == 2, p2, p3)));
 }
 OGJK_DEV int sa(const Simplex& s, int k) {
-  return k == 0 ? s.ia[0] : (k == 1 ? s.ia[1] : (k == 2 ? s.ia[2] : s.ia[3]));
+  const int a0 = keep_reg(s.ia[0]), a1 = keep_reg(s.ia[1]), a2 = keep_reg(s.ia[2]),
+            a3 = keep_reg(s.ia[3]);
+  return k == 0 ? a0 : (k == 1 ? a1 : (k == 2 ? a2 : a3));
 }
 OGJK_DEV int sb(const Simplex& s, int k) {
-  return k == 0 ? s.ib[0] : (k == 1 ? s.ib[1] : (k == 2 ? s.ib[2] : s.ib[3]));
+  const int b0 = keep_reg(s.ib[0]), b1 = keep_reg(s.ib[1]), b2 = keep_reg(s.ib[2]),
+            b3 = keep_reg(s.ib[3]);
+  return k == 0 ? b0 : (k == 1 ? b1 : (k == 2 ? b2 : b3));
 }
+// Write slot k: every slot is rewritten with a value select (no address select).
 OGJK_DEV void sset(Simplex& s, int k, D3 p, int a, int b) {
-  if (k == 0) {
-    s.p[0] = p; s.ia[0] = a; s.ib[0] = b;
-  } else if (k == 1) {
-    s.p[1] = p; s.ia[1] = a; s.ib[1] = b;
-  } else if (k == 2) {
-    s.p[2] = p; s.ia[2] = a; s.ib[2] = b;
-  } else {
-    s.p[3] = p; s.ia[3] = a; s.ib[3] = b;
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    const bool h = (k == t);
+    s.p[t] = sel(h, p, keep_reg(s.p[t]));
+    s.ia[t] = h ? a : keep_reg(s.ia[t]);
+    s.ib[t] = h ? b : keep_reg(s.ib[t]);
   }
 }
 template <int DST, int SRC>

@@ -38,8 +38,8 @@ struct OutArgs {
 // Doubles per pair in the optional FP64 verification record.
 constexpr int OGJK_F64_REC = 24;
 
-// Lanes per pair used by the launchers.
-constexpr int kLanesPerPair = 64;
+// Lanes per pair (16, 32 or 64) used by the launchers; returns -1 if invalid.
+int set_lanes_per_pair(int lanes);
 
 hipError_t launch_scene(int mode, const SceneArgs& s, const OutArgs& o, int n, hipStream_t st);
 hipError_t launch_scene_f64(int mode, const SceneArgs& s, const OutArgs& o, int n, hipStream_t st);

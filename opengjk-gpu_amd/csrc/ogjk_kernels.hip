@@ -146,7 +146,7 @@ OGJK_DEV void load_input(const OutArgs& o, int i, const Body<T, STRIDE>& A,
     const D3 e = sub(A.d(a), B.d(b));
     const bool same = (float)e.x == r->vrtx[k][0] && (float)e.y == r->vrtx[k][1] &&
                       (float)e.z == r->vrtx[k][2];
-    s.p[k] = same ? e : g;
+    s.p[k] = sel(same, e, g);
   }
   w.w0 = D3{(double)r->witnesses[0][0], (double)r->witnesses[0][1], (double)r->witnesses[0][2]};
   w.w1 = D3{(double)r->witnesses[1][0], (double)r->witnesses[1][1], (double)r->witnesses[1][2]};
@@ -223,23 +223,39 @@ __global__ void __launch_bounds__(256) k_body_r1(const float* __restrict__ x,
 
 // ----------------------------------------------------------------- launchers
 
-template <int MODE, class Src>
+template <int LPP, int MODE, class Src>
 static hipError_t launch_mode(const Src& src, const OutArgs& o, int n, hipStream_t st) {
-  constexpr int LPP = kLanesPerPair;
   constexpr int GPB = 256 / LPP;
   const unsigned blocks = (unsigned)((n + GPB - 1) / GPB);
   hipLaunchKernelGGL((k_pairs<LPP, MODE, Src>), dim3(blocks), dim3(256), 0, st, src, o, n);
   return hipGetLastError();
 }
 
+template <int LPP, class Src>
+static hipError_t launch_lpp(int mode, const Src& src, const OutArgs& o, int n, hipStream_t st) {
+  switch (mode) {
+    case 1: return launch_mode<LPP, 1>(src, o, n, st);
+    case 2: return launch_mode<LPP, 2>(src, o, n, st);
+    case 3: return launch_mode<LPP, 3>(src, o, n, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+static int g_lanes_per_pair = 64;
+
+int set_lanes_per_pair(int lanes) {
+  if (lanes != 16 && lanes != 32 && lanes != 64) return -1;
+  g_lanes_per_pair = lanes;
+  return 0;
+}
+
 template <class Src>
 static hipError_t launch_any(int mode, const Src& src, const OutArgs& o, int n, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  switch (mode) {
-    case 1: return launch_mode<1>(src, o, n, st);
-    case 2: return launch_mode<2>(src, o, n, st);
-    case 3: return launch_mode<3>(src, o, n, st);
-    default: return hipErrorInvalidValue;
+  switch (g_lanes_per_pair) {
+    case 16: return launch_lpp<16>(mode, src, o, n, st);
+    case 32: return launch_lpp<32>(mode, src, o, n, st);
+    default: return launch_lpp<64>(mode, src, o, n, st);
   }
 }
 

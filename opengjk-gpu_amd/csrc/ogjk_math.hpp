@@ -31,7 +31,7 @@ OGJK_DEV double norm2(D3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
 OGJK_DEV D3 cross(D3 a, D3 b) {
   return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
-OGJK_DEV D3 sel(bool c, D3 a, D3 b) { return c ? a : b; }
+OGJK_DEV D3 sel(bool c, D3 a, D3 b) { return D3{c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z}; }
 
 // ------------------------------------------------------------ collectives
 // A "group" is LPP consecutive lanes of one wave64 (LPP in {8,16,32,64}).

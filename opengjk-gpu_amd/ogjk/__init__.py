@@ -34,7 +34,8 @@ PAIR_DTYPE = np.dtype([("idx1", np.int32), ("idx2", np.int32)])
 
 # Every symbol include/opengjk/ogjk.h declares (checked by the ABI test).
 C_ABI_SYMBOLS = [
-    "ogjk_version", "ogjk_last_error", "ogjk_device_count", "ogjk_sync", "ogjk_pack_scene",
+    "ogjk_version", "ogjk_last_error", "ogjk_device_count", "ogjk_sync",
+    "ogjk_set_lanes_per_pair", "ogjk_pack_scene",
     "ogjk_collide_scene", "ogjk_pack_scene_f64", "ogjk_collide_scene_f64",
     "ogjk_collide_polytopes", "ogjk_collide_polytopes_indexed", "ogjk_collide_host",
     "ogjk_collide_host_indexed",
@@ -71,6 +72,7 @@ def lib():
             "ogjk_last_error": (C.c_char_p, []),
             "ogjk_device_count": (C.c_int, []),
             "ogjk_sync": (C.c_int, [vp]),
+            "ogjk_set_lanes_per_pair": (C.c_int, [C.c_int]),
             "ogjk_pack_scene": (C.c_int, [vp, vp, vp, i32, i64, vp, vp, vp, vp, vp]),
             "ogjk_pack_scene_f64": (C.c_int, [vp, i64, vp, vp, vp, vp]),
             "ogjk_collide_scene": (C.c_int, [C.c_int, C.POINTER(_Scene), vp, i32, vp, vp, vp, vp, vp]),
@@ -96,6 +98,10 @@ def _check(status: int) -> None:
 
 def _ptr(a):
     return None if a is None else a.ctypes.data
+
+
+def set_lanes_per_pair(lanes: int) -> None:
+    _check(lib().ogjk_set_lanes_per_pair(int(lanes)))
 
 
 def device_count() -> int:
