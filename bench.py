@@ -6,9 +6,10 @@ spheres of radius U[0.5,1.5] with centres in U[-1.29,1.29]^3 (about 60% of
 pairs penetrate), GJK then EPA for every pair, FP32 vertex storage with the
 FP32 support sweep + exact FP64 re-rank, FP64 sub-distance/EPA arithmetic.
 
-A step is one fused GJK+EPA launch over the whole batch, inputs resident in
-HBM as an SoA scene.  K steps are timed between device synchronisations;
-per-launch times come from HIP events recorded on the launch stream.
+A step is one GJK+EPA pass over the whole batch (GJK kernel, device worklist
+of penetrating pairs, persistent EPA kernel), inputs resident in HBM as an
+SoA scene.  K steps are timed between device synchronisations; per-step
+times come from HIP events recorded on the launch stream.
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -93,6 +94,10 @@ def main() -> int:
     ap.add_argument("--pairs", type=int, default=0, help="override the pair count")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-FP64 comparison")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run a 256 MiB elementwise copy (reads and writes 256 MiB) so a "
+                         "profiler's FETCH_SIZE/WRITE_SIZE can be scaled against known bytes")
+    ap.add_argument("--lanes", default="8,32", help="GJK,EPA lanes per pair")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -107,6 +112,14 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         return 1
+    lg, le = (int(x) for x in args.lanes.split(","))
+    ogjk.set_config(lg, le, False)
+    if args.calibrate:
+        x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        for _ in range(3):
+            y = x * 1.0
+        torch.cuda.synchronize()
+        del x, y
     wl = dict(WORKLOADS[args.workload])
     desc = wl.pop("desc")
     if args.pairs:
@@ -164,13 +177,14 @@ def main() -> int:
         "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": desc, "pairs": n, "vertices": int(batch.coords.shape[0]),
-                   "lanes_per_pair": 64, "layout": "SoA fp32 scene, fused GJK+EPA kernel"},
+                   "lanes_per_pair": {"gjk": lg, "epa": le},
+                   "layout": "SoA fp32 scene; GJK kernel -> device worklist -> persistent EPA kernel"},
         "step_ms_p50": float(np.percentile(ms, 50)), "step_ms_p99": float(np.percentile(ms, 99)),
         "plain_f64": plain,
         "speedup_vs_plain_f64": (value / plain["pairs_per_s"]) if plain else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_pairs<64,GJK+EPA,SceneSrc<float>>",
+                     "kernel": "k_gjk + k_epa_list (one step)",
                      "kernel_ms_mean": k_ms, "algorithmic_bytes_per_launch": algo},
     }
     if not args.no_cpu:

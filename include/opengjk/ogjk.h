@@ -24,6 +24,7 @@
 #ifndef OPENGJK_AMD_OGJK_H
 #define OPENGJK_AMD_OGJK_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #include "opengjk/common.h"
@@ -76,10 +77,15 @@ const char* ogjk_last_error(void);
 /* Number of visible HIP devices (0 without a GPU; never fails). */
 int ogjk_device_count(void);
 ogjk_status ogjk_sync(void* stream);
-/* Lanes of a wave64 that work on one pair: 16, 32 or 64 (default 64).
- * Process-wide setting read at launch time; not synchronised with launches
- * running on other threads. */
-ogjk_status ogjk_set_lanes_per_pair(int lanes);
+/* Launch shape, process-wide, read at launch time: lanes of a wave64 per
+ * pair in the GJK kernel (8, 16, 32, 64; default 8) and in the EPA kernel
+ * (16, 32, 64; default 32); fused != 0 runs GJK+EPA as one kernel instead of
+ * GJK -> device worklist -> persistent EPA.  Not synchronised with launches
+ * issued concurrently from other threads. */
+ogjk_status ogjk_set_config(int gjk_lanes, int epa_lanes, int fused);
+
+/* Device workspace bytes a GJK+EPA call over n_pairs needs. */
+size_t ogjk_workspace_bytes(int32_t n_pairs);
 
 /* ------------------------------------------------------------ scenes */
 
@@ -96,6 +102,14 @@ ogjk_status ogjk_pack_scene(const float* d_coords_aos, const int64_t* d_body_off
 ogjk_status ogjk_collide_scene(int mode, const ogjk_scene* scene, const int32_t* d_pairs,
                                int32_t n_pairs, gkSimplex* d_simplices, float* d_distances,
                                float* d_normals, double* d_f64, void* stream);
+
+/* Same, with a caller-provided device workspace (>= ogjk_workspace_bytes;
+ * used by OGJK_MODE_GJK_EPA).  Calls without it share one library-owned
+ * workspace per device and must not overlap on that device. */
+ogjk_status ogjk_collide_scene_ws(int mode, const ogjk_scene* scene, const int32_t* d_pairs,
+                                  int32_t n_pairs, gkSimplex* d_simplices, float* d_distances,
+                                  float* d_normals, double* d_f64, void* d_workspace,
+                                  size_t workspace_bytes, void* stream);
 
 /* Plain-FP64 comparison path: double storage, FP64 sweep, no candidate
  * re-rank; same selection rules and results (the denominator of the

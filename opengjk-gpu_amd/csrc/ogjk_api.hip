@@ -5,6 +5,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "ogjk_internal.hpp"
@@ -34,6 +35,33 @@ ogjk_status hip_fail(hipError_t e, const char* what) {
   } while (0)
 
 bool valid_mode(int m) { return m == OGJK_MODE_GJK || m == OGJK_MODE_EPA || m == OGJK_MODE_GJK_EPA; }
+
+// Library-owned GJK+EPA workspace, one per device, grown on demand.  Calls
+// that use it must not overlap on one device (pass an explicit workspace to
+// ogjk_collide_scene_ws for concurrent streams).
+struct CachedWorkspace {
+  std::mutex mu;
+  void* ptr[64] = {};
+  size_t bytes[64] = {};
+};
+CachedWorkspace g_ws;
+
+ogjk_status cached_workspace(size_t need, void** out) {
+  int dev = 0;
+  OGJK_HIP(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= 64) return fail(OGJK_ERR_UNSUPPORTED, "device id %d", dev);
+  std::lock_guard<std::mutex> lock(g_ws.mu);
+  if (g_ws.bytes[dev] < need) {
+    if (g_ws.ptr[dev]) OGJK_HIP(hipFree(g_ws.ptr[dev]), "hipFree(workspace)");
+    g_ws.ptr[dev] = nullptr;
+    g_ws.bytes[dev] = 0;
+    const size_t grow = need + need / 4;
+    OGJK_HIP(hipMalloc(&g_ws.ptr[dev], grow), "hipMalloc(workspace)");
+    g_ws.bytes[dev] = grow;
+  }
+  *out = g_ws.ptr[dev];
+  return OGJK_OK;
+}
 
 // One device allocation carved into 256-byte aligned pieces, freed on scope exit.
 struct DeviceArena {
@@ -110,7 +138,7 @@ ogjk_status run_staged(int mode, const HostScene& hs, const gkCollisionPair* pai
   size_t bytes = 3 * DeviceArena::up(nv * sizeof(float)) + DeviceArena::up(nb * 8) +
                  2 * DeviceArena::up(nb * 4) + DeviceArena::up((size_t)n * sizeof(gkSimplex)) +
                  DeviceArena::up((size_t)n * 4) + DeviceArena::up((size_t)n * 12) +
-                 DeviceArena::up((size_t)n * 8) + 256;
+                 DeviceArena::up((size_t)n * 8) + DeviceArena::up(ogjk::workspace_bytes(n)) + 256;
   OGJK_HIP(ar.reserve(bytes), "hipMalloc(staging)");
   float* dx = ar.take<float>(nv);
   float* dy = ar.take<float>(nv);
@@ -122,6 +150,7 @@ ogjk_status run_staged(int mode, const HostScene& hs, const gkCollisionPair* pai
   float* dd = ar.take<float>(n);
   float* dn = ar.take<float>(3 * (size_t)n);
   int32_t* dp = pairs ? ar.take<int32_t>(2 * (size_t)n) : nullptr;
+  void* dws = ar.take<char>(ogjk::workspace_bytes(n));
 
   OGJK_HIP(hipMemcpy(dx, hs.x.data(), nv * 4, hipMemcpyHostToDevice), "hipMemcpy(x)");
   OGJK_HIP(hipMemcpy(dy, hs.y.data(), nv * 4, hipMemcpyHostToDevice), "hipMemcpy(y)");
@@ -137,7 +166,7 @@ ogjk_status run_staged(int mode, const HostScene& hs, const gkCollisionPair* pai
   }
   const ogjk::SceneArgs sa{dx, dy, dz, doff, dcnt, dr1, (int32_t)nb, dp};
   const ogjk::OutArgs oa{ds, dd, dn, nullptr};
-  OGJK_HIP(ogjk::launch_scene(mode, sa, oa, n, nullptr), "kernel launch");
+  OGJK_HIP(ogjk::launch_scene(mode, sa, oa, n, dws, nullptr), "kernel launch");
   OGJK_HIP(hipDeviceSynchronize(), "kernel");
   OGJK_HIP(hipMemcpy(simplices, ds, (size_t)n * sizeof(gkSimplex), hipMemcpyDeviceToHost),
            "hipMemcpy(simplices)");
@@ -161,11 +190,18 @@ int ogjk_device_count(void) {
   return n;
 }
 
-ogjk_status ogjk_set_lanes_per_pair(int lanes) {
-  if (ogjk::set_lanes_per_pair(lanes) != 0)
-    return fail(OGJK_ERR_INVALID, "lanes per pair must be 16, 32 or 64 (got %d)", lanes);
+ogjk_status ogjk_set_config(int gjk_lanes, int epa_lanes, int fused) {
+  ogjk::LaunchConfig c;
+  c.gjk_lanes = gjk_lanes;
+  c.epa_lanes = epa_lanes;
+  c.fused = fused ? 1 : 0;
+  if (ogjk::set_config(c) != 0)
+    return fail(OGJK_ERR_INVALID, "lanes per pair: GJK 8/16/32/64, EPA 16/32/64 (got %d, %d)",
+                gjk_lanes, epa_lanes);
   return OGJK_OK;
 }
+
+size_t ogjk_workspace_bytes(int32_t n_pairs) { return ogjk::workspace_bytes(n_pairs); }
 
 ogjk_status ogjk_sync(void* stream) {
   OGJK_HIP(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
@@ -204,9 +240,10 @@ static ogjk_status check_out(int mode, int32_t n, gkSimplex* s, float* d, float*
   return OGJK_OK;
 }
 
-ogjk_status ogjk_collide_scene(int mode, const ogjk_scene* scene, const int32_t* d_pairs,
-                               int32_t n_pairs, gkSimplex* d_simplices, float* d_distances,
-                               float* d_normals, double* d_f64, void* stream) {
+ogjk_status ogjk_collide_scene_ws(int mode, const ogjk_scene* scene, const int32_t* d_pairs,
+                                  int32_t n_pairs, gkSimplex* d_simplices, float* d_distances,
+                                  float* d_normals, double* d_f64, void* d_workspace,
+                                  size_t workspace_bytes, void* stream) {
   const ogjk_status st = check_out(mode, n_pairs, d_simplices, d_distances, d_normals);
   if (st != OGJK_OK) return st;
   if (n_pairs == 0) return OGJK_OK;
@@ -216,11 +253,27 @@ ogjk_status ogjk_collide_scene(int mode, const ogjk_scene* scene, const int32_t*
   if (!d_pairs && (int64_t)scene->n_bodies < 2 * (int64_t)n_pairs)
     return fail(OGJK_ERR_INVALID, "implicit pairs need %lld bodies, scene has %d",
                 2 * (long long)n_pairs, scene->n_bodies);
+  if (mode == OGJK_MODE_GJK_EPA && (!d_workspace || workspace_bytes < ogjk::workspace_bytes(n_pairs)))
+    return fail(OGJK_ERR_INVALID, "workspace of %zu bytes needed", ogjk::workspace_bytes(n_pairs));
   const ogjk::SceneArgs sa{scene->x, scene->y, scene->z, scene->body_off, scene->body_cnt,
                            scene->body_r1, scene->n_bodies, d_pairs};
   const ogjk::OutArgs oa{d_simplices, d_distances, d_normals, d_f64};
-  OGJK_HIP(ogjk::launch_scene(mode, sa, oa, n_pairs, (hipStream_t)stream), "kernel launch");
+  OGJK_HIP(ogjk::launch_scene(mode, sa, oa, n_pairs, d_workspace, (hipStream_t)stream),
+           "kernel launch");
   return OGJK_OK;
+}
+
+ogjk_status ogjk_collide_scene(int mode, const ogjk_scene* scene, const int32_t* d_pairs,
+                               int32_t n_pairs, gkSimplex* d_simplices, float* d_distances,
+                               float* d_normals, double* d_f64, void* stream) {
+  void* ws = nullptr;
+  const size_t need = ogjk::workspace_bytes(n_pairs);
+  if (mode == OGJK_MODE_GJK_EPA && n_pairs > 0) {
+    const ogjk_status s2 = cached_workspace(need, &ws);
+    if (s2 != OGJK_OK) return s2;
+  }
+  return ogjk_collide_scene_ws(mode, scene, d_pairs, n_pairs, d_simplices, d_distances, d_normals,
+                               d_f64, ws, need, stream);
 }
 
 ogjk_status ogjk_collide_scene_f64(int mode, const ogjk_scene64* scene, const int32_t* d_pairs,
@@ -237,7 +290,12 @@ ogjk_status ogjk_collide_scene_f64(int mode, const ogjk_scene64* scene, const in
   const ogjk::SceneArgs sa{scene->x, scene->y, scene->z, scene->body_off, scene->body_cnt,
                            nullptr, scene->n_bodies, d_pairs};
   const ogjk::OutArgs oa{d_simplices, d_distances, d_normals, d_f64};
-  OGJK_HIP(ogjk::launch_scene_f64(mode, sa, oa, n_pairs, (hipStream_t)stream), "kernel launch");
+  void* ws = nullptr;
+  if (mode == OGJK_MODE_GJK_EPA) {
+    const ogjk_status s2 = cached_workspace(ogjk::workspace_bytes(n_pairs), &ws);
+    if (s2 != OGJK_OK) return s2;
+  }
+  OGJK_HIP(ogjk::launch_scene_f64(mode, sa, oa, n_pairs, ws, (hipStream_t)stream), "kernel launch");
   return OGJK_OK;
 }
 
@@ -250,7 +308,12 @@ ogjk_status ogjk_collide_polytopes(int mode, const gkPolytope* d_bd1, const gkPo
   if (!d_bd1 || !d_bd2) return fail(OGJK_ERR_INVALID, "NULL polytope array");
   const ogjk::PolyArgs pa{d_bd1, d_bd2, nullptr, 0};
   const ogjk::OutArgs oa{d_simplices, d_distances, d_normals, nullptr};
-  OGJK_HIP(ogjk::launch_poly(mode, pa, oa, n, (hipStream_t)stream), "kernel launch");
+  void* ws = nullptr;
+  if (mode == OGJK_MODE_GJK_EPA) {
+    const ogjk_status s2 = cached_workspace(ogjk::workspace_bytes(n), &ws);
+    if (s2 != OGJK_OK) return s2;
+  }
+  OGJK_HIP(ogjk::launch_poly(mode, pa, oa, n, ws, (hipStream_t)stream), "kernel launch");
   return OGJK_OK;
 }
 
@@ -265,7 +328,12 @@ ogjk_status ogjk_collide_polytopes_indexed(int mode, const gkPolytope* d_pool,
     return fail(OGJK_ERR_INVALID, "NULL pool/pairs or empty pool");
   const ogjk::PolyArgs pa{d_pool, nullptr, d_pairs, n_polytopes};
   const ogjk::OutArgs oa{d_simplices, d_distances, d_normals, nullptr};
-  OGJK_HIP(ogjk::launch_poly(mode, pa, oa, n_pairs, (hipStream_t)stream), "kernel launch");
+  void* ws = nullptr;
+  if (mode == OGJK_MODE_GJK_EPA) {
+    const ogjk_status s2 = cached_workspace(ogjk::workspace_bytes(n_pairs), &ws);
+    if (s2 != OGJK_OK) return s2;
+  }
+  OGJK_HIP(ogjk::launch_poly(mode, pa, oa, n_pairs, ws, (hipStream_t)stream), "kernel launch");
   return OGJK_OK;
 }
 

@@ -54,15 +54,39 @@ OGJK_DEV float body_r1(const Body<T, STRIDE>& b, int lane) {
 
 // --------------------------------------------------------------- support
 
+// Result of a support query: the selected vertex index and its coordinates
+// (exact FP64 conversion of the stored values).
+struct Sup {
+  int i;
+  D3 p;
+};
+
+// Finish an argmax: group reduction of each lane's (FP64 value, index, point)
+// candidate, then the reference's acceptance rule (see support below).
+template <int LPP>
+OGJK_DEV Sup finish_support(double bv, int bidx, D3 bp, D3 dir, bool has_inc, int inc_idx,
+                            D3 inc_pt) {
+  if (!(bv == bv)) bv = -INFINITY;
+  const int mine = bidx;
+  group_argmax_d<LPP>(bv, bidx);
+  if (has_inc && !(bv > dot(inc_pt, dir))) return Sup{inc_idx, inc_pt};
+  if (!has_inc && !(bv > -1e10)) return Sup{-1, D3{0.0, 0.0, 0.0}};
+  // the winner's coordinates live in the lane that owns index bidx
+  const int src = bidx & (LPP - 1);
+  (void)mine;
+  return Sup{bidx, D3{shfl_d<LPP>(bp.x, src), shfl_d<LPP>(bp.y, src), shfl_d<LPP>(bp.z, src)}};
+}
+
 // Exact emulation of the FP64 reference argmax of dot(x_i, dir) over one body.
 //   has_inc: GJK rule (GJK/cpu/openGJK.c:615-639): the incumbent support
 //            point keeps its place unless some vertex is strictly better; then
 //            the lowest index attaining the maximum wins.
 //   !has_inc: EPA rule (GJK/cpu/EPA.c:310-335): running max starts at -1e10,
-//            lowest index attaining the maximum wins; -1 if none beats -1e10.
+//            lowest index attaining the maximum wins; index -1 if none beats
+//            -1e10.
 // The result is group-uniform.
 template <int LPP, int STRIDE>
-OGJK_DEV int support(const Body<float, STRIDE>& b, D3 dir, bool has_inc, int inc_idx, D3 inc_pt,
+OGJK_DEV Sup support(const Body<float, STRIDE>& b, D3 dir, bool has_inc, int inc_idx, D3 inc_pt,
                      int lane) {
   // Scale the direction by an exact power of two so its max-norm is in
   // [0.5, 1): the FP32 copy then carries a relative error <= 2^-24 per
@@ -74,13 +98,18 @@ OGJK_DEV int support(const Body<float, STRIDE>& b, D3 dir, bool has_inc, int inc
   const float dx = (float)(dir.x * sc), dy = (float)(dir.y * sc), dz = (float)(dir.z * sc);
 
   float best = -INFINITY, second = -INFINITY;
+  float bx = 0.0f, by = 0.0f, bz = 0.0f;
   int bi = -1;
   for (int i = lane; i < b.n; i += LPP) {
-    const float d = fmaf(b.rx(i), dx, fmaf(b.ry(i), dy, b.rz(i) * dz));
+    const float px = b.rx(i), py = b.ry(i), pz = b.rz(i);
+    const float d = fmaf(px, dx, fmaf(py, dy, pz * dz));
     second = fmaxf(second, fminf(best, d));
     if (d > best) {
       best = d;
       bi = i;
+      bx = px;
+      by = py;
+      bz = pz;
     }
   }
   const float M = group_max_f<LPP>(best);
@@ -90,47 +119,49 @@ OGJK_DEV int support(const Body<float, STRIDE>& b, D3 dir, bool has_inc, int inc
 
   double bv = -INFINITY;
   int bidx = 0x7fffffff;
+  D3 bp{0.0, 0.0, 0.0};
   if (!multi) {
     if (bi >= 0 && best >= T) {
-      bv = dot(b.d(bi), dir);
+      bp = D3{(double)bx, (double)by, (double)bz};
+      bv = dot(bp, dir);
       bidx = bi;
     }
   } else {
     for (int i = lane; i < b.n; i += LPP) {
-      const float d = fmaf(b.rx(i), dx, fmaf(b.ry(i), dy, b.rz(i) * dz));
+      const float px = b.rx(i), py = b.ry(i), pz = b.rz(i);
+      const float d = fmaf(px, dx, fmaf(py, dy, pz * dz));
       if (d >= T) {
-        const double v = dot(b.d(i), dir);
+        const D3 p{(double)px, (double)py, (double)pz};
+        const double v = dot(p, dir);
         if (v > bv) {  // ascending i within the lane: keeps the lowest index
           bv = v;
           bidx = i;
+          bp = p;
         }
       }
     }
   }
-  if (!(bv == bv)) bv = -INFINITY;
-  group_argmax_d<LPP>(bv, bidx);
-  if (has_inc) return (bv > dot(inc_pt, dir)) ? bidx : inc_idx;
-  return (bv > -1e10) ? bidx : -1;
+  return finish_support<LPP>(bv, bidx, bp, dir, has_inc, inc_idx, inc_pt);
 }
 
 // Plain FP64 support over double storage: the same selection rules, computed
 // directly in FP64 (the comparison baseline; no FP32 pass).
 template <int LPP, int STRIDE>
-OGJK_DEV int support(const Body<double, STRIDE>& b, D3 dir, bool has_inc, int inc_idx, D3 inc_pt,
+OGJK_DEV Sup support(const Body<double, STRIDE>& b, D3 dir, bool has_inc, int inc_idx, D3 inc_pt,
                      int lane) {
   double bv = -INFINITY;
   int bidx = 0x7fffffff;
+  D3 bp{0.0, 0.0, 0.0};
   for (int i = lane; i < b.n; i += LPP) {
-    const double v = dot(b.d(i), dir);
+    const D3 p = b.d(i);
+    const double v = dot(p, dir);
     if (v > bv) {
       bv = v;
       bidx = i;
+      bp = p;
     }
   }
-  if (!(bv == bv)) bv = -INFINITY;
-  group_argmax_d<LPP>(bv, bidx);
-  if (has_inc) return (bv > dot(inc_pt, dir)) ? bidx : inc_idx;
-  return (bv > -1e10) ? bidx : -1;
+  return finish_support<LPP>(bv, bidx, bp, dir, has_inc, inc_idx, inc_pt);
 }
 
 // --------------------------------------------------------------- simplex
@@ -668,16 +699,12 @@ OGJK_DEV double gjk(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, Simplex&
   unsigned k = 0;
   do {
     k++;
-    const int na = support<LPP>(A, neg(v), true, ia, pa, lane);
-    if (na != ia) {
-      ia = na;
-      pa = A.d(ia);
-    }
-    const int nb = support<LPP>(B, v, true, ib, pb, lane);
-    if (nb != ib) {
-      ib = nb;
-      pb = B.d(ib);
-    }
+    const Sup sA = support<LPP>(A, neg(v), true, ia, pa, lane);
+    ia = sA.i;
+    pa = sA.p;
+    const Sup sB = support<LPP>(B, v, true, ib, pb, lane);
+    ib = sB.i;
+    pb = sB.p;
     const D3 wv = sub(pa, pb);
     const double gap = norm2(v) - dot(v, wv);
     if (gap <= eps_rel * norm2(v) || gap < eps_tot) break;

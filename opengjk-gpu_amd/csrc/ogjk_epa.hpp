@@ -27,7 +27,6 @@ constexpr int kEpaMaxVerts = kEpaMaxFaces + 4;
 struct EpaShared {
   double vx[kEpaMaxVerts], vy[kEpaMaxVerts], vz[kEpaMaxVerts];
   int via[kEpaMaxVerts], vib[kEpaMaxVerts];
-  unsigned short ekey[kEpaMaxFaces * 3];  // undirected edge key
   unsigned short edir[kEpaMaxFaces * 3];  // directed edge v1 | v2 << 8
   unsigned short surv[kEpaMaxFaces * 3];  // surviving horizon edges, in order
 };
@@ -35,6 +34,12 @@ struct EpaShared {
 OGJK_DEV void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
+}
+
+// Direction-free key of a packed edge (v1 | v2 << 8).
+OGJK_DEV int undirected(int e) {
+  const int a = e & 0xff, b = e >> 8;
+  return a < b ? (a | (b << 8)) : (b | (a << 8));
 }
 
 OGJK_DEV D3 lds_v(const EpaShared& sh, int i) { return D3{sh.vx[i], sh.vy[i], sh.vz[i]}; }
@@ -173,31 +178,36 @@ OGJK_DEV int closest_face(const Faces<FPL>& F, int lane, double& cdist) {
 template <int LPP, int FPL>
 OGJK_DEV void fetch_face(const Faces<FPL>& F, int slot, D3& n, int& verts) {
   const int q = slot / LPP, src = slot % LPP;
-  double x = F.nx[0], y = F.ny[0], z = F.nz[0];
-  int vv = F.verts[0];
+  // value selects over laundered registers (see keep_reg): no address select
+  double x = keep_reg(F.nx[0]), y = keep_reg(F.ny[0]), z = keep_reg(F.nz[0]);
+  int vv = keep_reg(F.verts[0]);
 #pragma unroll
   for (int t = 1; t < FPL; t++) {
-    if (q == t) {
-      x = F.nx[t];
-      y = F.ny[t];
-      z = F.nz[t];
-      vv = F.verts[t];
-    }
+    const bool h = (q == t);
+    x = h ? keep_reg(F.nx[t]) : x;
+    y = h ? keep_reg(F.ny[t]) : y;
+    z = h ? keep_reg(F.nz[t]) : z;
+    vv = h ? keep_reg(F.verts[t]) : vv;
   }
   n = D3{shfl_d<LPP>(x, src), shfl_d<LPP>(y, src), shfl_d<LPP>(z, src)};
   verts = shfl_i<LPP>(vv, src);
 }
 
+// support_epa (GJK/cpu/EPA.c:307-344): farthest vertex of A along dir and of
+// B along -dir; returns the Minkowski point A[ia] - B[ib].
 template <int LPP, class T, int STRIDE>
-OGJK_DEV void support_epa(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, D3 dir, int& ia, int& ib,
-                          int lane) {
+OGJK_DEV D3 support_epa(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, D3 dir, int& ia,
+                        int& ib, int lane) {
   const D3 zero{0.0, 0.0, 0.0};
-  ia = support<LPP>(A, dir, false, 0, zero, lane);
-  ib = support<LPP>(B, neg(dir), false, 0, zero, lane);
+  Sup a = support<LPP>(A, dir, false, 0, zero, lane);
+  Sup b = support<LPP>(B, neg(dir), false, 0, zero, lane);
   // The reference leaves its output unset when no vertex beats -1e10; such
   // coordinates are out of any physical range, fall back to vertex 0.
-  if (ia < 0) ia = 0;
-  if (ib < 0) ib = 0;
+  if (a.i < 0) a = Sup{0, A.d(0)};
+  if (b.i < 0) b = Sup{0, B.d(0)};
+  ia = a.i;
+  ib = b.i;
+  return sub(a.p, b.p);
 }
 
 // Is p farther than eps from every simplex vertex (GJK/cpu/EPA.c:453-463)?
@@ -254,8 +264,7 @@ OGJK_DEV void epa(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, Simplex& s
   if (s.n != 4) {
     int ia, ib;
     if (s.n == 1) {
-      support_epa<LPP>(A, B, s.p[0], ia, ib, lane);
-      const D3 nv = sub(A.d(ia), B.d(ib));
+      const D3 nv = support_epa<LPP>(A, B, s.p[0], ia, ib, lane);
       if (is_new_point(s, nv)) {
         sset(s, 1, nv, ia, ib);
         s.n = 2;
@@ -274,8 +283,7 @@ OGJK_DEV void epa(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, Simplex& s
         axis = D3{0.0, 0.0, 1.0};
         dir = cross(edge, axis);
       }
-      support_epa<LPP>(A, B, dir, ia, ib, lane);
-      const D3 nv = sub(A.d(ia), B.d(ib));
+      const D3 nv = support_epa<LPP>(A, B, dir, ia, ib, lane);
       if (is_new_point(s, nv)) {
         sset(s, 2, nv, ia, ib);
         s.n = 3;
@@ -286,8 +294,7 @@ OGJK_DEV void epa(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, Simplex& s
     }
     if (s.n == 3) {
       D3 dir = cross(sub(s.p[1], s.p[0]), sub(s.p[2], s.p[0]));
-      support_epa<LPP>(A, B, dir, ia, ib, lane);
-      D3 nv = sub(A.d(ia), B.d(ib));
+      D3 nv = support_epa<LPP>(A, B, dir, ia, ib, lane);
       if (is_new_point(s, nv)) {
         sset(s, 3, nv, ia, ib);
         s.n = 4;
@@ -295,8 +302,7 @@ OGJK_DEV void epa(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, Simplex& s
         dir = neg(dir);
       }
       if (s.n == 3) {
-        support_epa<LPP>(A, B, dir, ia, ib, lane);
-        nv = sub(A.d(ia), B.d(ib));
+        nv = support_epa<LPP>(A, B, dir, ia, ib, lane);
         if (is_new_point(s, nv)) {
           sset(s, 3, nv, ia, ib);
           s.n = 4;
@@ -367,8 +373,7 @@ OGJK_DEV void epa(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, Simplex& s
     fetch_face<LPP>(F, cf, n, cverts);
 
     int ia, ib;
-    support_epa<LPP>(A, B, n, ia, ib, lane);
-    const D3 nv = sub(A.d(ia), B.d(ib));
+    const D3 nv = support_epa<LPP>(A, B, n, ia, ib, lane);
     const double improvement = dot(n, nv) - cdist;
     if (improvement < tol) {
       face_result(A, B, sh, cverts, n, cdist, distance, w, normal);
@@ -427,9 +432,6 @@ OGJK_DEV void epa(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, Simplex& s
         sh.edir[e + 0] = (unsigned short)(x0 | (x1 << 8));
         sh.edir[e + 1] = (unsigned short)(x1 | (x2 << 8));
         sh.edir[e + 2] = (unsigned short)(x2 | (x0 << 8));
-        sh.ekey[e + 0] = (unsigned short)(min(x0, x1) | (max(x0, x1) << 8));
-        sh.ekey[e + 1] = (unsigned short)(min(x1, x2) | (max(x1, x2) << 8));
-        sh.ekey[e + 2] = (unsigned short)(min(x2, x0) | (max(x2, x0) << 8));
       }
     }
     wave_sync();
@@ -441,9 +443,10 @@ OGJK_DEV void epa(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, Simplex& s
       const int e = base + lane;
       bool keep = false;
       if (e < E) {
-        const unsigned short key = sh.ekey[e];
+        const int key = undirected(sh.edir[e]);
         int c = 0;
-        for (int f = 0; f < E; f++) c += (sh.ekey[f] == key);
+#pragma unroll 8
+        for (int f = 0; f < E; f++) c += (undirected(sh.edir[f]) == key);
         keep = (c == 1);
       }
       const uint64_t gb = group_ballot<LPP>(keep);

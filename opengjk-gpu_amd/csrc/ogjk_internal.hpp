@@ -38,12 +38,27 @@ struct OutArgs {
 // Doubles per pair in the optional FP64 verification record.
 constexpr int OGJK_F64_REC = 24;
 
-// Lanes per pair (16, 32 or 64) used by the launchers; returns -1 if invalid.
-int set_lanes_per_pair(int lanes);
+// Launch shape: lanes per pair for the GJK kernel (8..64) and the EPA kernel
+// (16..64), and whether GJK+EPA runs as one fused kernel instead of the
+// GJK -> worklist -> persistent EPA pipeline.
+struct LaunchConfig {
+  int gjk_lanes = 8;
+  int epa_lanes = 32;
+  int fused = 0;
+};
+int set_config(const LaunchConfig& c);  // -1 if invalid
+LaunchConfig get_config();
 
-hipError_t launch_scene(int mode, const SceneArgs& s, const OutArgs& o, int n, hipStream_t st);
-hipError_t launch_scene_f64(int mode, const SceneArgs& s, const OutArgs& o, int n, hipStream_t st);
-hipError_t launch_poly(int mode, const PolyArgs& p, const OutArgs& o, int n, hipStream_t st);
+// Device workspace a GJK+EPA launch over n pairs needs (counters + worklist).
+size_t workspace_bytes(int n);
+
+// ws: device workspace of workspace_bytes(n) (used by mode 3 only).
+hipError_t launch_scene(int mode, const SceneArgs& s, const OutArgs& o, int n, void* ws,
+                        hipStream_t st);
+hipError_t launch_scene_f64(int mode, const SceneArgs& s, const OutArgs& o, int n, void* ws,
+                            hipStream_t st);
+hipError_t launch_poly(int mode, const PolyArgs& p, const OutArgs& o, int n, void* ws,
+                       hipStream_t st);
 hipError_t launch_pack_f64(const float* aos, int64_t n_verts, double* x, double* y, double* z,
                            hipStream_t st);
 hipError_t launch_pack(const float* aos, const int64_t* off, const int32_t* cnt, int n_bodies,

@@ -62,18 +62,35 @@ OGJK_DEV int group_prefix(uint64_t gb) {
   return __popcll(gb & ((1ull << l) - 1ull));
 }
 
-template <int LPP>
-OGJK_DEV float shfl_xor_f(float v, int m) {
-  return __shfl_xor(v, m, LPP);
+// Exchange step m (1, 2, 4, ...) of an all-reduce over LPP lanes.  Steps
+// 1 and 2 are quad permutes, 4 and 8 are the row half-mirror and mirror
+// (lane i <-> 7-i, 15-i): applied in increasing m after the lower steps have
+// made each block of m lanes uniform they pair every block with its partner
+// block, like a butterfly, without leaving the VALU.  Steps >= 16 cross DPP
+// rows and go through ds_bpermute.
+template <int M>
+OGJK_DEV int xchg_i32(int v) {
+  if constexpr (M == 1) return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);
+  else if constexpr (M == 2) return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);
+  else if constexpr (M == 4) return __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);
+  else if constexpr (M == 8) return __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);
+  else return __shfl_xor(v, M, 64);
 }
-template <int LPP>
-OGJK_DEV int shfl_xor_i(int v, int m) {
-  return __shfl_xor(v, m, LPP);
+template <int M>
+OGJK_DEV float xchg(float v) {
+  return __builtin_bit_cast(float, xchg_i32<M>(__builtin_bit_cast(int, v)));
 }
-template <int LPP>
-OGJK_DEV double shfl_xor_d(double v, int m) {
-  return __shfl_xor(v, m, LPP);
+template <int M>
+OGJK_DEV int xchg(int v) {
+  return xchg_i32<M>(v);
 }
+template <int M>
+OGJK_DEV double xchg(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const int lo = xchg_i32<M>((int)(uint32_t)u), hi = xchg_i32<M>((int)(uint32_t)(u >> 32));
+  return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
 template <int LPP>
 OGJK_DEV double shfl_d(double v, int src) {
   return __shfl(v, src, LPP);
@@ -82,40 +99,47 @@ template <int LPP>
 OGJK_DEV int shfl_i(int v, int src) {
   return __shfl(v, src, LPP);
 }
-
 template <int LPP>
+OGJK_DEV float shfl_f(float v, int src) {
+  return __shfl(v, src, LPP);
+}
+
+template <int LPP, int M = 1>
 OGJK_DEV float group_max_f(float v) {
-#pragma unroll
-  for (int m = LPP / 2; m >= 1; m >>= 1) v = fmaxf(v, shfl_xor_f<LPP>(v, m));
-  return v;
+  if constexpr (M < LPP) {
+    v = fmaxf(v, xchg<M>(v));
+    return group_max_f<LPP, 2 * M>(v);
+  } else {
+    return v;
+  }
 }
 
 // Max of (value, index) with the lowest index winning ties.  NaN values must
 // already be mapped to -inf by the caller, so the result is group-uniform.
-template <int LPP>
+template <int LPP, int M = 1>
 OGJK_DEV void group_argmax_d(double& v, int& i) {
-#pragma unroll
-  for (int m = LPP / 2; m >= 1; m >>= 1) {
-    const double ov = shfl_xor_d<LPP>(v, m);
-    const int oi = shfl_xor_i<LPP>(i, m);
+  if constexpr (M < LPP) {
+    const double ov = xchg<M>(v);
+    const int oi = xchg<M>(i);
     if (ov > v || (ov == v && oi < i)) {
       v = ov;
       i = oi;
     }
+    group_argmax_d<LPP, 2 * M>(v, i);
   }
 }
 
 // Min of (value, index), lowest index on ties; NaN must be mapped to +inf.
-template <int LPP>
+template <int LPP, int M = 1>
 OGJK_DEV void group_argmin_d(double& v, int& i) {
-#pragma unroll
-  for (int m = LPP / 2; m >= 1; m >>= 1) {
-    const double ov = shfl_xor_d<LPP>(v, m);
-    const int oi = shfl_xor_i<LPP>(i, m);
+  if constexpr (M < LPP) {
+    const double ov = xchg<M>(v);
+    const int oi = xchg<M>(i);
     if (ov < v || (ov == v && oi < i)) {
       v = ov;
       i = oi;
     }
+    group_argmin_d<LPP, 2 * M>(v, i);
   }
 }
 

@@ -35,7 +35,7 @@ PAIR_DTYPE = np.dtype([("idx1", np.int32), ("idx2", np.int32)])
 # Every symbol include/opengjk/ogjk.h declares (checked by the ABI test).
 C_ABI_SYMBOLS = [
     "ogjk_version", "ogjk_last_error", "ogjk_device_count", "ogjk_sync",
-    "ogjk_set_lanes_per_pair", "ogjk_pack_scene",
+    "ogjk_set_config", "ogjk_workspace_bytes", "ogjk_collide_scene_ws", "ogjk_pack_scene",
     "ogjk_collide_scene", "ogjk_pack_scene_f64", "ogjk_collide_scene_f64",
     "ogjk_collide_polytopes", "ogjk_collide_polytopes_indexed", "ogjk_collide_host",
     "ogjk_collide_host_indexed",
@@ -72,7 +72,10 @@ def lib():
             "ogjk_last_error": (C.c_char_p, []),
             "ogjk_device_count": (C.c_int, []),
             "ogjk_sync": (C.c_int, [vp]),
-            "ogjk_set_lanes_per_pair": (C.c_int, [C.c_int]),
+            "ogjk_set_config": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+            "ogjk_workspace_bytes": (C.c_size_t, [i32]),
+            "ogjk_collide_scene_ws": (C.c_int, [C.c_int, C.POINTER(_Scene), vp, i32, vp, vp, vp, vp,
+                                                vp, C.c_size_t, vp]),
             "ogjk_pack_scene": (C.c_int, [vp, vp, vp, i32, i64, vp, vp, vp, vp, vp]),
             "ogjk_pack_scene_f64": (C.c_int, [vp, i64, vp, vp, vp, vp]),
             "ogjk_collide_scene": (C.c_int, [C.c_int, C.POINTER(_Scene), vp, i32, vp, vp, vp, vp, vp]),
@@ -100,8 +103,9 @@ def _ptr(a):
     return None if a is None else a.ctypes.data
 
 
-def set_lanes_per_pair(lanes: int) -> None:
-    _check(lib().ogjk_set_lanes_per_pair(int(lanes)))
+def set_config(gjk_lanes: int = 16, epa_lanes: int = 32, fused: bool = False) -> None:
+    """Launch shape (see ogjk_set_config in include/opengjk/ogjk.h)."""
+    _check(lib().ogjk_set_config(int(gjk_lanes), int(epa_lanes), int(bool(fused))))
 
 
 def device_count() -> int:

@@ -69,4 +69,6 @@ def test_invalid_arguments_fail_cleanly():
     assert b"mode" in lib.ogjk_last_error()
     assert lib.ogjk_collide_host(1, -3, None, None, None, None, None) == 1
     assert lib.ogjk_collide_host(1, 0, None, None, None, None, None) == 0  # n = 0: no-op
-    assert lib.ogjk_set_lanes_per_pair(24) == 1
+    assert lib.ogjk_set_config(24, 32, 0) == 1
+    assert lib.ogjk_set_config(16, 8, 0) == 1
+    assert lib.ogjk_set_config(16, 32, 0) == 0
