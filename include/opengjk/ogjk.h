@@ -84,7 +84,16 @@ ogjk_status ogjk_sync(void* stream);
  * issued concurrently from other threads. */
 ogjk_status ogjk_set_config(int gjk_lanes, int epa_lanes, int fused);
 
-/* Device workspace bytes a GJK+EPA call over n_pairs needs. */
+/* Size-bucketed launch shape: pairs with max(nA, nB) <= max_verts[b] (and
+ * above the previous bound) form bucket b; the last bucket is open-ended
+ * (its max_verts entry is ignored).  Each bucket runs GJK on gjk_lanes[b]
+ * (8, 16, 32, 64) and EPA on epa_lanes[b] (16, 32, 64) lanes per pair, all
+ * scheduled on the device (classify, scatter, persistent per-bucket kernels).
+ * Same process-wide scope as ogjk_set_config, which it replaces. */
+ogjk_status ogjk_set_buckets(int n_buckets, const int* max_verts, const int* gjk_lanes,
+                             const int* epa_lanes);
+
+/* Device workspace bytes a GJK or GJK+EPA call over n_pairs needs. */
 size_t ogjk_workspace_bytes(int32_t n_pairs);
 
 /* ------------------------------------------------------------ scenes */
@@ -104,7 +113,7 @@ ogjk_status ogjk_collide_scene(int mode, const ogjk_scene* scene, const int32_t*
                                float* d_normals, double* d_f64, void* stream);
 
 /* Same, with a caller-provided device workspace (>= ogjk_workspace_bytes;
- * used by OGJK_MODE_GJK_EPA).  Calls without it share one library-owned
+ * used by OGJK_MODE_GJK and OGJK_MODE_GJK_EPA).  Calls without it share one library-owned
  * workspace per device and must not overlap on that device. */
 ogjk_status ogjk_collide_scene_ws(int mode, const ogjk_scene* scene, const int32_t* d_pairs,
                                   int32_t n_pairs, gkSimplex* d_simplices, float* d_distances,

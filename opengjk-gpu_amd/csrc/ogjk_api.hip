@@ -192,12 +192,29 @@ int ogjk_device_count(void) {
 
 ogjk_status ogjk_set_config(int gjk_lanes, int epa_lanes, int fused) {
   ogjk::LaunchConfig c;
-  c.gjk_lanes = gjk_lanes;
-  c.epa_lanes = epa_lanes;
+  c.nb = 1;
+  c.gjk_lanes[0] = gjk_lanes;
+  c.epa_lanes[0] = epa_lanes;
   c.fused = fused ? 1 : 0;
   if (ogjk::set_config(c) != 0)
     return fail(OGJK_ERR_INVALID, "lanes per pair: GJK 8/16/32/64, EPA 16/32/64 (got %d, %d)",
                 gjk_lanes, epa_lanes);
+  return OGJK_OK;
+}
+
+ogjk_status ogjk_set_buckets(int n_buckets, const int* max_verts, const int* gjk_lanes,
+                             const int* epa_lanes) {
+  if (n_buckets < 1 || n_buckets > ogjk::kMaxBuckets || !max_verts || !gjk_lanes || !epa_lanes)
+    return fail(OGJK_ERR_INVALID, "1..%d buckets with non-NULL tables", ogjk::kMaxBuckets);
+  ogjk::LaunchConfig c;
+  c.nb = n_buckets;
+  for (int b = 0; b < n_buckets; b++) {
+    c.max_verts[b] = b < n_buckets - 1 ? max_verts[b] : 0x7fffffff;
+    c.gjk_lanes[b] = gjk_lanes[b];
+    c.epa_lanes[b] = epa_lanes[b];
+  }
+  if (ogjk::set_config(c) != 0)
+    return fail(OGJK_ERR_INVALID, "bucket bounds must increase; lanes GJK 8/16/32/64, EPA 16/32/64");
   return OGJK_OK;
 }
 
@@ -253,7 +270,7 @@ ogjk_status ogjk_collide_scene_ws(int mode, const ogjk_scene* scene, const int32
   if (!d_pairs && (int64_t)scene->n_bodies < 2 * (int64_t)n_pairs)
     return fail(OGJK_ERR_INVALID, "implicit pairs need %lld bodies, scene has %d",
                 2 * (long long)n_pairs, scene->n_bodies);
-  if (mode == OGJK_MODE_GJK_EPA && (!d_workspace || workspace_bytes < ogjk::workspace_bytes(n_pairs)))
+  if (mode != OGJK_MODE_EPA && (!d_workspace || workspace_bytes < ogjk::workspace_bytes(n_pairs)))
     return fail(OGJK_ERR_INVALID, "workspace of %zu bytes needed", ogjk::workspace_bytes(n_pairs));
   const ogjk::SceneArgs sa{scene->x, scene->y, scene->z, scene->body_off, scene->body_cnt,
                            scene->body_r1, scene->n_bodies, d_pairs};
@@ -268,7 +285,7 @@ ogjk_status ogjk_collide_scene(int mode, const ogjk_scene* scene, const int32_t*
                                float* d_normals, double* d_f64, void* stream) {
   void* ws = nullptr;
   const size_t need = ogjk::workspace_bytes(n_pairs);
-  if (mode == OGJK_MODE_GJK_EPA && n_pairs > 0) {
+  if (mode != OGJK_MODE_EPA && n_pairs > 0) {
     const ogjk_status s2 = cached_workspace(need, &ws);
     if (s2 != OGJK_OK) return s2;
   }
@@ -291,7 +308,7 @@ ogjk_status ogjk_collide_scene_f64(int mode, const ogjk_scene64* scene, const in
                            nullptr, scene->n_bodies, d_pairs};
   const ogjk::OutArgs oa{d_simplices, d_distances, d_normals, d_f64};
   void* ws = nullptr;
-  if (mode == OGJK_MODE_GJK_EPA) {
+  if (mode != OGJK_MODE_EPA) {
     const ogjk_status s2 = cached_workspace(ogjk::workspace_bytes(n_pairs), &ws);
     if (s2 != OGJK_OK) return s2;
   }
@@ -309,7 +326,7 @@ ogjk_status ogjk_collide_polytopes(int mode, const gkPolytope* d_bd1, const gkPo
   const ogjk::PolyArgs pa{d_bd1, d_bd2, nullptr, 0};
   const ogjk::OutArgs oa{d_simplices, d_distances, d_normals, nullptr};
   void* ws = nullptr;
-  if (mode == OGJK_MODE_GJK_EPA) {
+  if (mode != OGJK_MODE_EPA) {
     const ogjk_status s2 = cached_workspace(ogjk::workspace_bytes(n), &ws);
     if (s2 != OGJK_OK) return s2;
   }
@@ -329,7 +346,7 @@ ogjk_status ogjk_collide_polytopes_indexed(int mode, const gkPolytope* d_pool,
   const ogjk::PolyArgs pa{d_pool, nullptr, d_pairs, n_polytopes};
   const ogjk::OutArgs oa{d_simplices, d_distances, d_normals, nullptr};
   void* ws = nullptr;
-  if (mode == OGJK_MODE_GJK_EPA) {
+  if (mode != OGJK_MODE_EPA) {
     const ogjk_status s2 = cached_workspace(ogjk::workspace_bytes(n_pairs), &ws);
     if (s2 != OGJK_OK) return s2;
   }

@@ -61,107 +61,165 @@ struct Sup {
   D3 p;
 };
 
-// Finish an argmax: group reduction of each lane's (FP64 value, index, point)
-// candidate, then the reference's acceptance rule (see support below).
-template <int LPP>
-OGJK_DEV Sup finish_support(double bv, int bidx, D3 bp, D3 dir, bool has_inc, int inc_idx,
-                            D3 inc_pt) {
-  if (!(bv == bv)) bv = -INFINITY;
-  const int mine = bidx;
-  group_argmax_d<LPP>(bv, bidx);
-  if (has_inc && !(bv > dot(inc_pt, dir))) return Sup{inc_idx, inc_pt};
-  if (!has_inc && !(bv > -1e10)) return Sup{-1, D3{0.0, 0.0, 0.0}};
-  // the winner's coordinates live in the lane that owns index bidx
-  const int src = bidx & (LPP - 1);
-  (void)mine;
-  return Sup{bidx, D3{shfl_d<LPP>(bp.x, src), shfl_d<LPP>(bp.y, src), shfl_d<LPP>(bp.z, src)}};
-}
-
-// Exact emulation of the FP64 reference argmax of dot(x_i, dir) over one body.
-//   has_inc: GJK rule (GJK/cpu/openGJK.c:615-639): the incumbent support
-//            point keeps its place unless some vertex is strictly better; then
-//            the lowest index attaining the maximum wins.
-//   !has_inc: EPA rule (GJK/cpu/EPA.c:310-335): running max starts at -1e10,
+// One support query.  has_inc selects the reference's rule:
+//   has_inc: GJK (GJK/cpu/openGJK.c:615-639): the incumbent support point
+//            keeps its place unless some vertex is strictly better; then the
+//            lowest index attaining the maximum wins.
+//   !has_inc: EPA (GJK/cpu/EPA.c:310-335): running max starts at -1e10, the
 //            lowest index attaining the maximum wins; index -1 if none beats
 //            -1e10.
-// The result is group-uniform.
-template <int LPP, int STRIDE>
-OGJK_DEV Sup support(const Body<float, STRIDE>& b, D3 dir, bool has_inc, int inc_idx, D3 inc_pt,
-                     int lane) {
-  // Scale the direction by an exact power of two so its max-norm is in
-  // [0.5, 1): the FP32 copy then carries a relative error <= 2^-24 per
-  // component whatever the magnitude of dir (tiny GJK directions included).
+struct Query {
+  D3 dir;
+  bool has_inc;
+  int inc_idx;
+  D3 inc_pt;
+};
+
+// A lane's FP64-exact candidate after the sweep: value, index, point.
+struct Cand {
+  double v;
+  int i;
+  D3 p;
+};
+
+// FP32 sweep state of one lane.
+struct Sweep32 {
+  float dx, dy, dz;
+  float best, second, bx, by, bz;
+  int bi;
+};
+
+// Scale the direction by an exact power of two so its max-norm is in
+// [0.5, 1): the FP32 copy then carries a relative error <= 2^-24 per
+// component whatever the magnitude of dir (tiny GJK directions included).
+OGJK_DEV Sweep32 sweep32_begin(D3 dir) {
   const double mx = fmax(fabs(dir.x), fmax(fabs(dir.y), fabs(dir.z)));
   int e = 0;
   (void)frexp(mx, &e);
   const double sc = ldexp(1.0, -e);
-  const float dx = (float)(dir.x * sc), dy = (float)(dir.y * sc), dz = (float)(dir.z * sc);
+  Sweep32 s;
+  s.dx = (float)(dir.x * sc);
+  s.dy = (float)(dir.y * sc);
+  s.dz = (float)(dir.z * sc);
+  s.best = s.second = -INFINITY;
+  s.bx = s.by = s.bz = 0.0f;
+  s.bi = -1;
+  return s;
+}
 
-  float best = -INFINITY, second = -INFINITY;
-  float bx = 0.0f, by = 0.0f, bz = 0.0f;
-  int bi = -1;
-  for (int i = lane; i < b.n; i += LPP) {
-    const float px = b.rx(i), py = b.ry(i), pz = b.rz(i);
-    const float d = fmaf(px, dx, fmaf(py, dy, pz * dz));
-    second = fmaxf(second, fminf(best, d));
-    if (d > best) {
-      best = d;
-      bi = i;
-      bx = px;
-      by = py;
-      bz = pz;
-    }
+template <int STRIDE>
+OGJK_DEV void sweep32_step(const Body<float, STRIDE>& b, int i, Sweep32& s) {
+  const float px = b.rx(i), py = b.ry(i), pz = b.rz(i);
+  const float d = fmaf(px, s.dx, fmaf(py, s.dy, pz * s.dz));
+  s.second = fmaxf(s.second, fminf(s.best, d));
+  if (d > s.best) {
+    s.best = d;
+    s.bi = i;
+    s.bx = px;
+    s.by = py;
+    s.bz = pz;
   }
-  const float M = group_max_f<LPP>(best);
-  // |D32 - sc*D64| <= 16u * r1 (u = 2^-24): candidates lie within 2x that.
-  const float T = M - 0x1.0p-19f * b.r1;
-  const bool multi = group_any<LPP>(second >= T);
+}
 
-  double bv = -INFINITY;
-  int bidx = 0x7fffffff;
-  D3 bp{0.0, 0.0, 0.0};
+// Candidates within T of the FP32 maximum, re-ranked in FP64 with the
+// reference's association.  `multi`: some lane of the group has more than one
+// candidate, so every lane rescans its vertices inside the window.
+template <int LPP, int STRIDE>
+OGJK_DEV Cand candidates32(const Body<float, STRIDE>& b, const Sweep32& s, float T, bool multi,
+                           D3 dir, int lane) {
+  Cand c{-INFINITY, 0x7fffffff, D3{0.0, 0.0, 0.0}};
   if (!multi) {
-    if (bi >= 0 && best >= T) {
-      bp = D3{(double)bx, (double)by, (double)bz};
-      bv = dot(bp, dir);
-      bidx = bi;
+    if (s.bi >= 0 && s.best >= T) {
+      c.p = D3{(double)s.bx, (double)s.by, (double)s.bz};
+      c.v = dot(c.p, dir);
+      c.i = s.bi;
     }
   } else {
     for (int i = lane; i < b.n; i += LPP) {
       const float px = b.rx(i), py = b.ry(i), pz = b.rz(i);
-      const float d = fmaf(px, dx, fmaf(py, dy, pz * dz));
+      const float d = fmaf(px, s.dx, fmaf(py, s.dy, pz * s.dz));
       if (d >= T) {
         const D3 p{(double)px, (double)py, (double)pz};
         const double v = dot(p, dir);
-        if (v > bv) {  // ascending i within the lane: keeps the lowest index
-          bv = v;
-          bidx = i;
-          bp = p;
+        if (v > c.v) {  // ascending i within the lane keeps the lowest index
+          c.v = v;
+          c.i = i;
+          c.p = p;
         }
       }
     }
   }
-  return finish_support<LPP>(bv, bidx, bp, dir, has_inc, inc_idx, inc_pt);
+  if (!(c.v == c.v)) c.v = -INFINITY;
+  return c;
 }
 
-// Plain FP64 support over double storage: the same selection rules, computed
+// Group argmax of the lanes' candidates (lowest index on ties), then the
+// acceptance rule of the query and the winner's point.  Usual case: one lane
+// holds the only candidate and a single parallel shuffle fetches it.
+template <int LPP>
+OGJK_DEV Sup finish1(Cand c, const Query& q) {
+  const uint64_t holders = group_ballot<LPP>(c.v > -INFINITY);
+  int src;
+  if (__popcll(holders) == 1) {
+    src = (int)__builtin_ctzll(holders);
+    c.v = shfl_d<LPP>(c.v, src);
+    c.i = shfl_i<LPP>(c.i, src);
+  } else {
+    group_argmax_d<LPP>(c.v, c.i);
+    src = c.i & (LPP - 1);  // lane l sweeps indices l, l+LPP, ...
+  }
+  if (q.has_inc && !(c.v > dot(q.inc_pt, q.dir))) return Sup{q.inc_idx, q.inc_pt};
+  if (!q.has_inc && !(c.v > -1e10)) return Sup{-1, D3{0.0, 0.0, 0.0}};
+  return Sup{c.i, D3{shfl_d<LPP>(c.p.x, src), shfl_d<LPP>(c.p.y, src), shfl_d<LPP>(c.p.z, src)}};
+}
+
+// Support point of one body (FP32 sweep, FP64 re-rank of the window).
+template <int LPP, int STRIDE>
+OGJK_DEV Sup support(const Body<float, STRIDE>& A, const Query& q, int lane) {
+  Sweep32 sw = sweep32_begin(q.dir);
+  for (int i = lane; i < A.n; i += LPP) sweep32_step(A, i, sw);
+  const float m = group_max_f<LPP>(sw.best);
+  // |D32 - scale*D64| <= 16u * r1 (u = 2^-24): candidates lie within 2x that.
+  const float t = m - 0x1.0p-19f * A.r1;
+  const bool multi = group_any<LPP>(sw.second >= t);
+  return finish1<LPP>(candidates32<LPP>(A, sw, t, multi, q.dir, lane), q);
+}
+
+// The support points of both bodies, one query after the other.  (Sweeping
+// the two bodies in one loop keeps two sweep states live and costs more in
+// registers than it gains in overlap.)
+template <int LPP, int STRIDE>
+OGJK_DEV void support_pair(const Body<float, STRIDE>& A, const Query& qa,
+                           const Body<float, STRIDE>& B, const Query& qb, int lane, Sup& ra,
+                           Sup& rb) {
+  ra = support<LPP>(A, qa, lane);
+  rb = support<LPP>(B, qb, lane);
+}
+
+// Plain FP64 path over double storage: the same selection rules computed
 // directly in FP64 (the comparison baseline; no FP32 pass).
 template <int LPP, int STRIDE>
-OGJK_DEV Sup support(const Body<double, STRIDE>& b, D3 dir, bool has_inc, int inc_idx, D3 inc_pt,
-                     int lane) {
-  double bv = -INFINITY;
-  int bidx = 0x7fffffff;
-  D3 bp{0.0, 0.0, 0.0};
-  for (int i = lane; i < b.n; i += LPP) {
-    const D3 p = b.d(i);
-    const double v = dot(p, dir);
-    if (v > bv) {
-      bv = v;
-      bidx = i;
-      bp = p;
+OGJK_DEV void support_pair(const Body<double, STRIDE>& A, const Query& qa,
+                           const Body<double, STRIDE>& B, const Query& qb, int lane, Sup& ra,
+                           Sup& rb) {
+  Cand ca{-INFINITY, 0x7fffffff, D3{0.0, 0.0, 0.0}}, cb = ca;
+  const int nmax = A.n > B.n ? A.n : B.n;
+  for (int i = lane; i < nmax; i += LPP) {
+    if (i < A.n) {
+      const D3 p = A.d(i);
+      const double v = dot(p, qa.dir);
+      if (v > ca.v) ca = Cand{v, i, p};
+    }
+    if (i < B.n) {
+      const D3 p = B.d(i);
+      const double v = dot(p, qb.dir);
+      if (v > cb.v) cb = Cand{v, i, p};
     }
   }
-  return finish_support<LPP>(bv, bidx, bp, dir, has_inc, inc_idx, inc_pt);
+  if (!(ca.v == ca.v)) ca.v = -INFINITY;
+  if (!(cb.v == cb.v)) cb.v = -INFINITY;
+  ra = finish1<LPP>(ca, qa);
+  rb = finish1<LPP>(cb, qb);
 }
 
 // --------------------------------------------------------------- simplex
@@ -699,10 +757,10 @@ OGJK_DEV double gjk(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, Simplex&
   unsigned k = 0;
   do {
     k++;
-    const Sup sA = support<LPP>(A, neg(v), true, ia, pa, lane);
+    Sup sA, sB;
+    support_pair<LPP>(A, Query{neg(v), true, ia, pa}, B, Query{v, true, ib, pb}, lane, sA, sB);
     ia = sA.i;
     pa = sA.p;
-    const Sup sB = support<LPP>(B, v, true, ib, pb, lane);
     ib = sB.i;
     pb = sB.p;
     const D3 wv = sub(pa, pb);

@@ -199,8 +199,9 @@ template <int LPP, class T, int STRIDE>
 OGJK_DEV D3 support_epa(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, D3 dir, int& ia,
                         int& ib, int lane) {
   const D3 zero{0.0, 0.0, 0.0};
-  Sup a = support<LPP>(A, dir, false, 0, zero, lane);
-  Sup b = support<LPP>(B, neg(dir), false, 0, zero, lane);
+  Sup a, b;
+  support_pair<LPP>(A, Query{dir, false, 0, zero}, B, Query{neg(dir), false, 0, zero}, lane, a,
+                    b);
   // The reference leaves its output unset when no vertex beats -1e10; such
   // coordinates are out of any physical range, fall back to vertex 0.
   if (a.i < 0) a = Sup{0, A.d(0)};

@@ -38,13 +38,25 @@ struct OutArgs {
 // Doubles per pair in the optional FP64 verification record.
 constexpr int OGJK_F64_REC = 24;
 
-// Launch shape: lanes per pair for the GJK kernel (8..64) and the EPA kernel
-// (16..64), and whether GJK+EPA runs as one fused kernel instead of the
-// GJK -> worklist -> persistent EPA pipeline.
+// Launch shape.  Pairs are bucketed by max(nA, nB): bucket b holds sizes
+// <= max_verts[b] (the last bucket is open-ended) and runs GJK on
+// gjk_lanes[b] (8, 16, 32, 64) lanes per pair and EPA on epa_lanes[b] (16,
+// 32, 64).  fused != 0 runs GJK+EPA as one kernel per pair instead of the
+// GJK -> worklist -> persistent EPA pipeline (bucket 0's EPA lanes).
+constexpr int kMaxBuckets = 4;
 struct LaunchConfig {
-  int gjk_lanes = 8;
-  int epa_lanes = 32;
+  int nb = 1;
+  int max_verts[kMaxBuckets] = {0x7fffffff, 0x7fffffff, 0x7fffffff, 0x7fffffff};
+  int gjk_lanes[kMaxBuckets] = {8, 8, 8, 8};
+  int epa_lanes[kMaxBuckets] = {32, 32, 32, 32};
   int fused = 0;
+};
+
+// Per-launch copy of the bucket table passed to the kernels.
+struct BucketPlan {
+  int nb;
+  int max_verts[kMaxBuckets];
+  int epa_class[kMaxBuckets];  // 0, 1, 2 for EPA on 16, 32, 64 lanes
 };
 int set_config(const LaunchConfig& c);  // -1 if invalid
 LaunchConfig get_config();

@@ -29,6 +29,13 @@ struct SceneSrc {
   const int32_t* pairs;
   using BodyT = Body<T, 1>;
 
+  // max(nA, nB) of pair i, or 0 for an invalid pair (it goes to bucket 0)
+  OGJK_DEV int pair_size(int i) const {
+    const int a = pairs ? pairs[2 * i] : 2 * i, b = pairs ? pairs[2 * i + 1] : 2 * i + 1;
+    if (a < 0 || a >= nb || b < 0 || b >= nb) return 0;
+    return cnt[a] > cnt[b] ? cnt[a] : cnt[b];
+  }
+
   template <int LPP>
   OGJK_DEV bool load(int i, int lane, BodyT& A, BodyT& B) const {
     int a, b;
@@ -53,6 +60,20 @@ struct PolySrc {
   const gkCollisionPair* pairs;
   int32_t npool;
   using BodyT = Body<float, 3>;
+
+  OGJK_DEV int pair_size(int i) const {
+    const gkPolytope *pa, *pb;
+    if (pairs) {
+      const int ia = pairs[i].idx1, ib = pairs[i].idx2;
+      if (ia < 0 || ia >= npool || ib < 0 || ib >= npool) return 0;
+      pa = a + ia;
+      pb = a + ib;
+    } else {
+      pa = a + i;
+      pb = b + i;
+    }
+    return pa->numpoints > pb->numpoints ? pa->numpoints : pb->numpoints;
+  }
 
   template <int LPP>
   OGJK_DEV bool load(int i, int lane, BodyT& A, BodyT& B) const {
@@ -237,16 +258,130 @@ __global__ void __launch_bounds__(256) k_pairs(Src src, OutArgs out, int n) {
   if (lane == 0) store(out, pair, s, w, dist, normal, kEpa);
 }
 
-// GJK for every pair.  With EMIT (GJK then EPA), separated pairs are finished
-// here (contact normal from the witnesses, GJK/cpu/EPA.c:370-373) and each
-// penetrating pair is appended to the EPA worklist with its FP64 state.
+// ------------------------------------------------- size-bucketed pipeline
+//
+// Workspace (see workspace_bytes): 8 KiB of counters, then the pair list
+// (int32, bucket b at [start_b, start_b + count_b)), then the EPA jobs (one
+// 192-B record per penetrating pair, grouped by EPA lane class).
+//   counters[0..3]   pairs per bucket        counters[4..7]   scatter cursors
+//   counters[12..14] jobs per EPA class
+//   counters[64 + (q*8 + s)*32]  claim cursor of slice s of queue q
+//                    (q 0..3: GJK buckets, q 4..6: EPA classes)
+enum { kCntBucket = 0, kCntScatter = 4, kCntJobs = 12 };
+
+OGJK_DEV int bucket_of(const BucketPlan& p, int size) {
+  int b = 0;
+#pragma unroll
+  for (int k = 0; k < kMaxBuckets - 1; k++)
+    if (k < p.nb - 1 && size > p.max_verts[k]) b = k + 1;
+  return b;
+}
+
+OGJK_DEV unsigned bucket_start(const unsigned* c, int b) {
+  unsigned s = 0;
+  for (int k = 0; k < b; k++) s += __atomic_load_n(&c[kCntBucket + k], __ATOMIC_RELAXED);
+  return s;
+}
+
+// First job slot of EPA class `cls`: classes are laid out in order, each sized
+// by the pairs of the buckets that feed it.
+OGJK_DEV unsigned class_start(const unsigned* c, const BucketPlan& p, int cls) {
+  unsigned s = 0;
+  for (int k = 0; k < p.nb; k++)
+    if (p.epa_class[k] < cls) s += __atomic_load_n(&c[kCntBucket + k], __ATOMIC_RELAXED);
+  return s;
+}
+
+// Device-scope atomics on one address serialise across the XCDs, so every
+// counter is bumped once per wave: ballot, one atomicAdd of the popcount by
+// the lowest lane, broadcast, and a per-lane prefix.  Returns this lane's slot
+// (meaningful where pred).  Call with the wave's active lanes converged.
+OGJK_DEV unsigned wave_fetch_add(unsigned* ctr, bool pred) {
+  const uint64_t m = __ballot(pred);
+  if (m == 0) return 0u;
+  const int leader = (int)__builtin_ctzll(m);
+  unsigned base = 0u;
+  if (lane_id() == leader) base = atomicAdd(ctr, (unsigned)__popcll(m));
+  base = (unsigned)__shfl((int)base, leader, 64);
+  return base + (unsigned)__popcll(m & ((1ull << lane_id()) - 1ull));
+}
+
+// Work-claim cursors.  One device-scope atomic address serialises every
+// claim, so each queue (4 GJK buckets, 3 EPA classes) is split into kSlices
+// contiguous slices with one cursor each, on its own 128-B line.  A wave
+// starts on the slice of its block and moves on when a slice runs dry.
+constexpr int kSlices = 8;
+constexpr int kLineU = 32;  // uints per 128-B line
+constexpr int kClaimBase = 64;
+constexpr int kQueueEpa = 4;
+constexpr size_t kCounterBytes = 8192;  // >= (kClaimBase + 7*kSlices*kLineU) * 4
+
+// Claim the next item of [0, count) for every lane with `want` (group
+// leaders).  Returns the item, or 0xffffffff when all slices are exhausted.
+// Wave-uniform state: slice, tried.
+OGJK_DEV unsigned claim(unsigned* c, int queue, unsigned count, int& slice, int& tried,
+                        bool want) {
+  unsigned result = 0xffffffffu;
+  bool need = want;
+  slice = __builtin_amdgcn_readfirstlane(slice);
+  tried = __builtin_amdgcn_readfirstlane(tried);
+  while (__ballot(need) != 0 && tried < kSlices) {
+    const unsigned lo = (unsigned)(((uint64_t)count * slice) / kSlices);
+    const unsigned hi = (unsigned)(((uint64_t)count * (slice + 1)) / kSlices);
+    unsigned* cur = &c[kClaimBase + (queue * kSlices + slice) * kLineU];
+    // a drained slice is seen with a load; only live slices take an atomic
+    const bool live =
+        __builtin_amdgcn_readfirstlane(__atomic_load_n(cur, __ATOMIC_RELAXED)) < hi - lo;
+    const unsigned k = live ? wave_fetch_add(cur, need) : 0xffffffffu;
+    if (need && live && lo + k < hi) {
+      result = lo + k;
+      need = false;
+    }
+    if (__ballot(need) != 0) {
+      slice = (slice + 1) & (kSlices - 1);
+      tried++;
+    }
+  }
+  return result;
+}
+
+template <class Src>
+__global__ void __launch_bounds__(256) k_classify(Src src, int n, BucketPlan plan, unsigned* c) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int b = i < n ? bucket_of(plan, src.pair_size(i)) : -1;
+  for (int k = 0; k < plan.nb; k++) (void)wave_fetch_add(&c[kCntBucket + k], b == k);
+}
+
+template <class Src>
+__global__ void __launch_bounds__(256) k_scatter(Src src, int n, BucketPlan plan, unsigned* c,
+                                                 int* list) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int b = i < n ? bucket_of(plan, src.pair_size(i)) : -1;
+  for (int k = 0; k < plan.nb; k++) {
+    const unsigned slot = wave_fetch_add(&c[kCntScatter + k], b == k);
+    if (b == k) list[bucket_start(c, k) + slot] = i;
+  }
+}
+
+// GJK over one bucket, one group per list entry and one wave per block: the
+// hardware dispatcher balances pairs of different sizes wave by wave, and no
+// wave waits for the slowest pair of a larger block.  Blocks past the bucket's
+// count (the grid is sized for every pair) return at once.  With EMIT (GJK
+// then EPA) separated pairs are finished here (contact normal from the
+// witnesses, GJK/cpu/EPA.c:370-373) and each penetrating pair is appended to
+// its EPA class's job list with its FP64 state.  With a single bucket the
+// list is the identity and is not materialised.
 template <int LPP, bool EMIT, class Src>
-__global__ void __launch_bounds__(256, OGJK_GJK_WAVES) k_gjk(Src src, OutArgs out, int n, unsigned* counters,
-                                             EpaJob* jobs) {
-  constexpr int GPB = 256 / LPP;
+__global__ void __launch_bounds__(64, OGJK_GJK_WAVES)
+    k_gjk(Src src, OutArgs out, int n, BucketPlan plan, int bucket, unsigned* c, const int* list,
+          EpaJob* jobs) {
+  constexpr int GPB = 64 / LPP;
   const int lane = threadIdx.x & (LPP - 1);
-  const int pair = blockIdx.x * GPB + (int)(threadIdx.x / LPP);
-  if (pair >= n) return;
+  const unsigned count = __builtin_amdgcn_readfirstlane(
+      plan.nb > 1 ? __atomic_load_n(&c[kCntBucket + bucket], __ATOMIC_RELAXED) : (unsigned)n);
+  const unsigned j = blockIdx.x * GPB + threadIdx.x / LPP;
+  if (j >= count) return;
+  const int pair = plan.nb > 1 ? list[bucket_start(c, bucket) + j] : (int)j;
   typename Src::BodyT A, B;
   const bool ok = src.template load<LPP>(pair, lane, A, B) && A.n > 0 && B.n > 0;
   if (!ok) {
@@ -256,35 +391,40 @@ __global__ void __launch_bounds__(256, OGJK_GJK_WAVES) k_gjk(Src src, OutArgs ou
   Simplex s = {};
   Wit w = {};
   const double dist = gjk<LPP>(A, B, s, w, lane);
+  const bool pen = EMIT && !(dist > kEps64);
+  const int cls = plan.epa_class[bucket];
+  const unsigned k = wave_fetch_add(&c[kCntJobs + cls], lane == 0 && pen);
   if (lane != 0) return;
-  if (EMIT && !(dist > kEps64)) {
-    const unsigned j = atomicAdd(&counters[0], 1u);
-    job_store(jobs + j, pair, s, w, dist);
-    return;
+  if (pen) {
+    job_store(jobs + class_start(c, plan, cls) + k, pair, s, w, dist);
+  } else {
+    const D3 normal = EMIT ? contact_normal(w.w0, w.w1) : D3{0.0, 0.0, 0.0};
+    store(out, pair, s, w, dist, normal, EMIT);
   }
-  const D3 normal = EMIT ? contact_normal(w.w0, w.w1) : D3{0.0, 0.0, 0.0};
-  store(out, pair, s, w, dist, normal, EMIT);
 }
 
-// EPA over the worklist, persistent: each group claims jobs from an atomic
-// cursor until the list (length counters[0]) is exhausted.
+// EPA over one class's job list, persistent: the groups of a wave claim jobs
+// together from sliced cursors until the list is exhausted.
 template <int LPP, class Src>
-__global__ void __launch_bounds__(256, OGJK_EPA_WAVES) k_epa_list(Src src, OutArgs out, unsigned* counters,
-                                                  const EpaJob* jobs) {
+__global__ void __launch_bounds__(256, OGJK_EPA_WAVES)
+    k_epa_list(Src src, OutArgs out, BucketPlan plan, int cls, unsigned* c, const EpaJob* jobs) {
   constexpr int GPB = 256 / LPP;
   __shared__ EpaShared sh[GPB];
   const int lane = threadIdx.x & (LPP - 1);
   const int g = threadIdx.x / LPP;
-  const unsigned njobs = __atomic_load_n(&counters[0], __ATOMIC_RELAXED);
+  const unsigned njobs =
+      __builtin_amdgcn_readfirstlane(__atomic_load_n(&c[kCntJobs + cls], __ATOMIC_RELAXED));
+  if (blockIdx.x * GPB >= njobs) return;
+  const EpaJob* mine = jobs + __builtin_amdgcn_readfirstlane(class_start(c, plan, cls));
+  int slice = blockIdx.x & (kSlices - 1), tried = 0;
   for (;;) {
-    unsigned j = 0;
-    if (lane == 0) j = atomicAdd(&counters[1], 1u);
+    unsigned j = claim(c, kQueueEpa + cls, njobs, slice, tried, lane == 0);
     j = (unsigned)shfl_i<LPP>((int)j, 0);
     if (j >= njobs) break;
     Simplex s = {};
     Wit w = {};
     double dist;
-    const int pair = job_load(jobs + j, s, w, dist);
+    const int pair = job_load(mine + j, s, w, dist);
     typename Src::BodyT A, B;
     src.template load<LPP>(pair, lane, A, B);  // validated by the GJK kernel
     D3 normal{0.0, 0.0, 0.0};
@@ -293,8 +433,8 @@ __global__ void __launch_bounds__(256, OGJK_EPA_WAVES) k_epa_list(Src src, OutAr
   }
 }
 
-__global__ void k_zero(unsigned* counters) {
-  if (threadIdx.x < 2) counters[threadIdx.x] = 0u;
+__global__ void __launch_bounds__(256) k_zero(unsigned* c) {
+  for (unsigned i = threadIdx.x; i < kCounterBytes / 4; i += 256) c[i] = 0u;
 }
 
 // AoS pool -> SoA arrays, one thread per vertex.
@@ -338,16 +478,39 @@ __global__ void __launch_bounds__(256) k_body_r1(const float* __restrict__ x,
 
 static LaunchConfig g_cfg;
 
+static bool lanes_ok(int l, bool epa) {
+  return (!epa && l == 8) || l == 16 || l == 32 || l == 64;
+}
+
 int set_config(const LaunchConfig& c) {
-  const bool gok = c.gjk_lanes == 8 || c.gjk_lanes == 16 || c.gjk_lanes == 32 || c.gjk_lanes == 64;
-  const bool eok = c.epa_lanes == 16 || c.epa_lanes == 32 || c.epa_lanes == 64;
-  if (!gok || !eok) return -1;
+  if (c.nb < 1 || c.nb > kMaxBuckets) return -1;
+  for (int b = 0; b < c.nb; b++) {
+    if (!lanes_ok(c.gjk_lanes[b], false) || !lanes_ok(c.epa_lanes[b], true)) return -1;
+    if (b > 0 && c.max_verts[b] <= c.max_verts[b - 1] && b < c.nb - 1) return -1;
+  }
   g_cfg = c;
   return 0;
 }
 LaunchConfig get_config() { return g_cfg; }
 
-size_t workspace_bytes(int n) { return 256 + (size_t)(n > 0 ? n : 0) * sizeof(EpaJob); }
+static int epa_class_of_lanes(int lanes) { return lanes == 16 ? 0 : (lanes == 32 ? 1 : 2); }
+
+static BucketPlan plan_of(const LaunchConfig& c) {
+  BucketPlan p{};
+  p.nb = c.nb;
+  for (int b = 0; b < kMaxBuckets; b++) {
+    p.max_verts[b] = b < c.nb ? c.max_verts[b] : 0x7fffffff;
+    p.epa_class[b] = b < c.nb ? epa_class_of_lanes(c.epa_lanes[b]) : 0;
+  }
+  return p;
+}
+
+static size_t up256(size_t b) { return (b + 255) & ~size_t(255); }
+
+size_t workspace_bytes(int n) {
+  const size_t m = (size_t)(n > 0 ? n : 0);
+  return kCounterBytes + up256(4 * m) + m * sizeof(EpaJob);
+}
 
 template <int LPP, int MODE, class Src>
 static hipError_t launch_pairs(const Src& src, const OutArgs& o, int n, hipStream_t st) {
@@ -357,74 +520,101 @@ static hipError_t launch_pairs(const Src& src, const OutArgs& o, int n, hipStrea
   return hipGetLastError();
 }
 
+// Resident grid for a persistent launch over at most n items.
+static unsigned persistent_blocks(int n, int gpb, unsigned cap = 4096u) {
+  const unsigned want = (unsigned)((n + gpb - 1) / gpb);
+  return want < cap ? want : cap;
+}
+
 template <int LPP, bool EMIT, class Src>
-static hipError_t launch_gjk(const Src& src, const OutArgs& o, int n, void* ws, hipStream_t st) {
-  constexpr int GPB = 256 / LPP;
-  const unsigned blocks = (unsigned)((n + GPB - 1) / GPB);
-  unsigned* counters = (unsigned*)ws;
-  EpaJob* jobs = (EpaJob*)((char*)ws + 256);
-  hipLaunchKernelGGL((k_gjk<LPP, EMIT, Src>), dim3(blocks), dim3(256), 0, st, src, o, n, counters,
-                     jobs);
+static hipError_t launch_gjk(const Src& src, const OutArgs& o, int n, const BucketPlan& p, int b,
+                             void* ws, hipStream_t st) {
+  unsigned* c = (unsigned*)ws;
+  const int* list = (const int*)((char*)ws + kCounterBytes);
+  EpaJob* jobs = (EpaJob*)((char*)ws + kCounterBytes + up256(4 * (size_t)n));
+  constexpr int GPB = 64 / LPP;
+  hipLaunchKernelGGL((k_gjk<LPP, EMIT, Src>), dim3((unsigned)((n + GPB - 1) / GPB)), dim3(64), 0,
+                     st, src, o, n, p, b, c, list, jobs);
   return hipGetLastError();
 }
 
 template <int LPP, class Src>
-static hipError_t launch_epa_list(const Src& src, const OutArgs& o, int n, void* ws,
-                                  hipStream_t st) {
-  constexpr int GPB = 256 / LPP;
-  // enough resident blocks to fill the device; the groups pull jobs until done
-  const unsigned want = (unsigned)((n + GPB - 1) / GPB);
-  const unsigned blocks = want < 4096u ? want : 4096u;
-  unsigned* counters = (unsigned*)ws;
-  const EpaJob* jobs = (const EpaJob*)((char*)ws + 256);
-  hipLaunchKernelGGL((k_epa_list<LPP, Src>), dim3(blocks), dim3(256), 0, st, src, o, counters, jobs);
+static hipError_t launch_epa_list(const Src& src, const OutArgs& o, int n, const BucketPlan& p,
+                                  int cls, void* ws, hipStream_t st) {
+  unsigned* c = (unsigned*)ws;
+  const EpaJob* jobs = (const EpaJob*)((char*)ws + kCounterBytes + up256(4 * (size_t)n));
+  hipLaunchKernelGGL((k_epa_list<LPP, Src>), dim3(persistent_blocks(n, 256 / LPP)), dim3(256), 0,
+                     st, src, o, p, cls, c, jobs);
   return hipGetLastError();
 }
 
-template <class Src>
-static hipError_t gjk_by_lanes(bool emit, const Src& src, const OutArgs& o, int n, void* ws,
-                               hipStream_t st) {
-  switch (g_cfg.gjk_lanes) {
-    case 8: return emit ? launch_gjk<8, true>(src, o, n, ws, st) : launch_gjk<8, false>(src, o, n, ws, st);
-    case 16: return emit ? launch_gjk<16, true>(src, o, n, ws, st) : launch_gjk<16, false>(src, o, n, ws, st);
-    case 32: return emit ? launch_gjk<32, true>(src, o, n, ws, st) : launch_gjk<32, false>(src, o, n, ws, st);
-    default: return emit ? launch_gjk<64, true>(src, o, n, ws, st) : launch_gjk<64, false>(src, o, n, ws, st);
+template <bool EMIT, class Src>
+static hipError_t gjk_by_lanes(int lanes, const Src& src, const OutArgs& o, int n,
+                               const BucketPlan& p, int b, void* ws, hipStream_t st) {
+  switch (lanes) {
+    case 8: return launch_gjk<8, EMIT>(src, o, n, p, b, ws, st);
+    case 16: return launch_gjk<16, EMIT>(src, o, n, p, b, ws, st);
+    case 32: return launch_gjk<32, EMIT>(src, o, n, p, b, ws, st);
+    default: return launch_gjk<64, EMIT>(src, o, n, p, b, ws, st);
   }
 }
 
 template <int MODE, class Src>
 static hipError_t pairs_by_lanes(const Src& src, const OutArgs& o, int n, hipStream_t st) {
-  switch (g_cfg.epa_lanes) {
+  switch (g_cfg.epa_lanes[0]) {
     case 16: return launch_pairs<16, MODE>(src, o, n, st);
     case 32: return launch_pairs<32, MODE>(src, o, n, st);
     default: return launch_pairs<64, MODE>(src, o, n, st);
   }
 }
 
+// GJK (mode 1) or GJK -> EPA (mode 3) over size buckets; every step is a
+// device-driven kernel, so the sequence can be captured in a graph.
 template <class Src>
-static hipError_t epa_list_by_lanes(const Src& src, const OutArgs& o, int n, void* ws,
-                                    hipStream_t st) {
-  switch (g_cfg.epa_lanes) {
-    case 16: return launch_epa_list<16>(src, o, n, ws, st);
-    case 32: return launch_epa_list<32>(src, o, n, ws, st);
-    default: return launch_epa_list<64>(src, o, n, ws, st);
+static hipError_t launch_bucketed(int mode, const Src& src, const OutArgs& o, int n, void* ws,
+                                  hipStream_t st) {
+  const LaunchConfig cfg = g_cfg;
+  const BucketPlan p = plan_of(cfg);
+  unsigned* c = (unsigned*)ws;
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(256), 0, st, c);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (p.nb > 1) {
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL((k_classify<Src>), dim3(blocks), dim3(256), 0, st, src, n, p, c);
+    hipLaunchKernelGGL((k_scatter<Src>), dim3(blocks), dim3(256), 0, st, src, n, p, c,
+                       (int*)((char*)ws + kCounterBytes));
+    if ((e = hipGetLastError()) != hipSuccess) return e;
   }
+  for (int b = 0; b < p.nb; b++) {
+    e = mode == 3 ? gjk_by_lanes<true>(cfg.gjk_lanes[b], src, o, n, p, b, ws, st)
+                  : gjk_by_lanes<false>(cfg.gjk_lanes[b], src, o, n, p, b, ws, st);
+    if (e != hipSuccess) return e;
+  }
+  if (mode != 3) return hipSuccess;
+  bool done[3] = {false, false, false};
+  for (int b = 0; b < p.nb; b++) {
+    const int cls = p.epa_class[b];
+    if (done[cls]) continue;
+    done[cls] = true;
+    switch (cfg.epa_lanes[b]) {
+      case 16: e = launch_epa_list<16>(src, o, n, p, cls, ws, st); break;
+      case 32: e = launch_epa_list<32>(src, o, n, p, cls, ws, st); break;
+      default: e = launch_epa_list<64>(src, o, n, p, cls, ws, st); break;
+    }
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 template <class Src>
 static hipError_t launch_any(int mode, const Src& src, const OutArgs& o, int n, void* ws,
                              hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  if (mode == 1) return gjk_by_lanes(false, src, o, n, ws, st);
   if (mode == 2) return pairs_by_lanes<2>(src, o, n, st);
-  if (mode != 3) return hipErrorInvalidValue;
-  if (g_cfg.fused) return pairs_by_lanes<3>(src, o, n, st);
-  hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, (unsigned*)ws);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  e = gjk_by_lanes(true, src, o, n, ws, st);
-  if (e != hipSuccess) return e;
-  return epa_list_by_lanes(src, o, n, ws, st);
+  if (mode != 1 && mode != 3) return hipErrorInvalidValue;
+  if (mode == 3 && g_cfg.fused) return pairs_by_lanes<3>(src, o, n, st);
+  return launch_bucketed(mode, src, o, n, ws, st);
 }
 
 hipError_t launch_scene(int mode, const SceneArgs& s, const OutArgs& o, int n, void* ws,

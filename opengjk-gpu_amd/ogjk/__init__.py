@@ -35,7 +35,7 @@ PAIR_DTYPE = np.dtype([("idx1", np.int32), ("idx2", np.int32)])
 # Every symbol include/opengjk/ogjk.h declares (checked by the ABI test).
 C_ABI_SYMBOLS = [
     "ogjk_version", "ogjk_last_error", "ogjk_device_count", "ogjk_sync",
-    "ogjk_set_config", "ogjk_workspace_bytes", "ogjk_collide_scene_ws", "ogjk_pack_scene",
+    "ogjk_set_config", "ogjk_set_buckets", "ogjk_workspace_bytes", "ogjk_collide_scene_ws", "ogjk_pack_scene",
     "ogjk_collide_scene", "ogjk_pack_scene_f64", "ogjk_collide_scene_f64",
     "ogjk_collide_polytopes", "ogjk_collide_polytopes_indexed", "ogjk_collide_host",
     "ogjk_collide_host_indexed",
@@ -73,6 +73,7 @@ def lib():
             "ogjk_device_count": (C.c_int, []),
             "ogjk_sync": (C.c_int, [vp]),
             "ogjk_set_config": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+            "ogjk_set_buckets": (C.c_int, [C.c_int, vp, vp, vp]),
             "ogjk_workspace_bytes": (C.c_size_t, [i32]),
             "ogjk_collide_scene_ws": (C.c_int, [C.c_int, C.POINTER(_Scene), vp, i32, vp, vp, vp, vp,
                                                 vp, C.c_size_t, vp]),
@@ -106,6 +107,15 @@ def _ptr(a):
 def set_config(gjk_lanes: int = 16, epa_lanes: int = 32, fused: bool = False) -> None:
     """Launch shape (see ogjk_set_config in include/opengjk/ogjk.h)."""
     _check(lib().ogjk_set_config(int(gjk_lanes), int(epa_lanes), int(bool(fused))))
+
+
+def set_buckets(max_verts, gjk_lanes, epa_lanes) -> None:
+    """Size-bucketed launch shape (see ogjk_set_buckets in include/opengjk/ogjk.h)."""
+    nb = len(gjk_lanes)
+    mv = np.array(list(max_verts) + [0] * (nb - len(max_verts)), dtype=np.int32)
+    gl = np.array(gjk_lanes, dtype=np.int32)
+    el = np.array(epa_lanes, dtype=np.int32)
+    _check(lib().ogjk_set_buckets(nb, _ptr(mv), _ptr(gl), _ptr(el)))
 
 
 def device_count() -> int:
