@@ -34,6 +34,22 @@ WORKLOADS = {
                              desc="1000000 pairs, vertex counts log-uniform in [4,4096], "
                                   "ellipsoids axes U[0.3,2] rotated, centres U[-1.5,1.5]^3, GJK then EPA"),
 }
+# lanes per pair (GJK, EPA) used when --lanes is not given
+DEFAULT_LANES = {"sphere64": "8,32", "ellipsoid_uneven": "64,64"}
+# rocprofv3 summary holding the measured HBM traffic of one step of the workload
+PROFILE = {"sphere64": "gjk_epa_sphere64_100k.md", "ellipsoid_uneven": "gjk_epa_ellipsoid_uneven_1m.md"}
+
+
+def measured_traffic(workload: str):
+    """HBM bytes per step from the committed rocprofv3 summary (FETCH_SIZE and
+    WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", PROFILE.get(workload, ""))
+    if not os.path.isfile(path):
+        return None
+    for line in open(path):
+        if line.strip().startswith("traffic_bytes_per_step:"):
+            return float(line.split(":", 1)[1].strip())
+    return None
 
 
 def bytes_per_launch(batch, mode_epa=True) -> float:
@@ -97,7 +113,7 @@ def main() -> int:
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 256 MiB elementwise copy (reads and writes 256 MiB) so a "
                          "profiler's FETCH_SIZE/WRITE_SIZE can be scaled against known bytes")
-    ap.add_argument("--lanes", default="8,32", help="GJK,EPA lanes per pair")
+    ap.add_argument("--lanes", default=None, help="GJK,EPA lanes per pair (default per workload)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -112,7 +128,8 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         return 1
-    lg, le = (int(x) for x in args.lanes.split(","))
+    lanes = args.lanes or DEFAULT_LANES[args.workload]
+    lg, le = (int(x) for x in lanes.split(","))
     ogjk.set_config(lg, le, False)
     if args.calibrate:
         x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
@@ -160,12 +177,33 @@ def main() -> int:
     wall, ms = timed(False)
     print(f"# mixed: {wall / args.steps * 1e3:.3f} ms/step wall, kernel median {np.median(ms):.3f} ms",
           file=sys.stderr, flush=True)
-    plain = None
+    plain = pipe64 = None
     if not args.no_plain:
-        wall_p, ms_p = timed(True)
-        plain = {"pairs_per_s": n * args.steps / wall_p, "ms_per_step": wall_p / args.steps * 1e3,
-                 "kernel_ms_median": float(np.median(ms_p))}
-        print(f"# plain fp64: {wall_p / args.steps * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
+        # the same split pipeline over FP64 vertex storage (isolates the FP32 sweep)
+        wall_q, ms_q = timed(True)
+        pipe64 = {"pairs_per_s": n * args.steps / wall_q, "ms_per_step": wall_q / args.steps * 1e3}
+        print(f"# pipeline over fp64 storage: {pipe64['ms_per_step']:.3f} ms/step", file=sys.stderr,
+              flush=True)
+        # plain FP64 per-pair kernel: one group per pair runs GJK then EPA in one
+        # non-persistent launch over FP64 SoA vertices, no worklist; the faster
+        # of 32 and 64 lanes per pair is the denominator
+        plain = {"variants": {}}
+        for el in (32, 64):
+            ogjk.set_config(lg, el, True)
+            wall_p, ms_p = timed(True)
+            plain["variants"][f"{el}_lanes"] = {"pairs_per_s": n * args.steps / wall_p,
+                                                "ms_per_step": wall_p / args.steps * 1e3}
+            print(f"# plain fp64 per-pair kernel, {el} lanes: {wall_p / args.steps * 1e3:.3f} ms/step",
+                  file=sys.stderr, flush=True)
+        # the plain kernel's results equal the mixed pipeline's, pair by pair
+        chk = scene.outputs(n, f64=True)
+        scene.collide(3, n, out=chk, plain_f64=True, f64_record=True)
+        torch.cuda.synchronize()
+        plain["identical_to_mixed_fraction"] = float((chk["f64"] == out["f64"]).all(dim=1).float().mean())
+        del chk
+        ogjk.set_config(lg, le, False)
+        best = max(plain["variants"].values(), key=lambda v: v["pairs_per_s"])
+        plain.update(best)
 
     value = n * args.steps / wall
     algo = bytes_per_launch(batch)
@@ -182,8 +220,9 @@ def main() -> int:
         "step_ms_p50": float(np.percentile(ms, 50)), "step_ms_p99": float(np.percentile(ms, 99)),
         "plain_f64": plain,
         "speedup_vs_plain_f64": (value / plain["pairs_per_s"]) if plain else None,
+        "pipeline_f64_storage": pipe64,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
                      "kernel": "k_gjk + k_epa_list (one step)",
                      "kernel_ms_mean": k_ms, "algorithmic_bytes_per_launch": algo},
     }
