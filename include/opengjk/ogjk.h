@@ -46,6 +46,23 @@ enum { OGJK_MODE_GJK = 1, OGJK_MODE_EPA = 2, OGJK_MODE_GJK_EPA = 3 };
 /* Device-resident structure-of-arrays scene.  Body b owns vertices
  * [body_off[b], body_off[b] + body_cnt[b]) of x/y/z; body_r1[b] >=
  * max |x|+|y|+|z| over its vertices (written by ogjk_pack_scene). */
+/* Vertex clusters of a scene (optional; see ogjk_build_clusters).  Each body's
+ * vertices are copied in order of direction from the body's centre and cut
+ * into clusters of 32 slots with a bounding sphere and box each; support sweeps over bodies of at least min_verts
+ * vertices then skip every cluster that cannot hold the maximum.  Results are
+ * identical with and without clusters.  All arrays are device memory. */
+typedef struct ogjk_clusters {
+  float* x;            /* n_slots: vertices in cluster order */
+  float* y;
+  float* z;
+  int32_t* orig;       /* n_slots: body-local input index of each slot (-1: padding) */
+  int64_t* body_slot;  /* n_bodies: first slot of each body (a multiple of 32) */
+  float* bounds;       /* 8 * n_slots / 32: per cluster centre x, y, z, radius,
+                          half extents x, y, z, unused */
+  int64_t n_slots;     /* capacity of x, y, z, orig: >= ogjk_cluster_slots() */
+  int32_t min_verts;   /* smaller bodies are swept without culling */
+} ogjk_clusters;
+
 typedef struct ogjk_scene {
   const float* x;
   const float* y;
@@ -54,6 +71,7 @@ typedef struct ogjk_scene {
   const int32_t* body_cnt;
   const float* body_r1;
   int32_t n_bodies;
+  const ogjk_clusters* clusters; /* host pointer, or NULL: no culling */
 } ogjk_scene;
 
 /* Plain-FP64 scene (double coordinates) for the comparison path. */
@@ -119,6 +137,17 @@ ogjk_status ogjk_collide_scene_ws(int mode, const ogjk_scene* scene, const int32
                                   int32_t n_pairs, gkSimplex* d_simplices, float* d_distances,
                                   float* d_normals, double* d_f64, void* d_workspace,
                                   size_t workspace_bytes, void* stream);
+
+/* Slots a cluster build needs for a scene of n_verts vertices in n_bodies
+ * bodies (padding to 32 per body included). */
+int64_t ogjk_cluster_slots(int64_t n_verts, int32_t n_bodies);
+
+/* Builds the clusters of every body of a packed scene (scene->x/y/z,
+ * body_off, body_cnt) into caller-allocated arrays.  Synchronous on stream;
+ * allocates scratch for the per-body sort.  No counterpart in the reference:
+ * an acceleration structure for the support function
+ * (GJK/cpu/openGJK.c:615-639, GJK/cpu/EPA.c:307-344) on large hulls. */
+ogjk_status ogjk_build_clusters(const ogjk_scene* scene, ogjk_clusters* clusters, void* stream);
 
 /* Plain-FP64 comparison path: double storage, FP64 sweep, no candidate
  * re-rank; same selection rules and results (the denominator of the

@@ -272,11 +272,45 @@ ogjk_status ogjk_collide_scene_ws(int mode, const ogjk_scene* scene, const int32
                 2 * (long long)n_pairs, scene->n_bodies);
   if (mode != OGJK_MODE_EPA && (!d_workspace || workspace_bytes < ogjk::workspace_bytes(n_pairs)))
     return fail(OGJK_ERR_INVALID, "workspace of %zu bytes needed", ogjk::workspace_bytes(n_pairs));
-  const ogjk::SceneArgs sa{scene->x, scene->y, scene->z, scene->body_off, scene->body_cnt,
-                           scene->body_r1, scene->n_bodies, d_pairs};
+  ogjk::SceneArgs sa{scene->x, scene->y, scene->z, scene->body_off, scene->body_cnt,
+                     scene->body_r1, scene->n_bodies, d_pairs};
+  if (const ogjk_clusters* cl = scene->clusters) {
+    if (!cl->x || !cl->y || !cl->z || !cl->orig || !cl->body_slot || !cl->bounds)
+      return fail(OGJK_ERR_INVALID, "incomplete clusters");
+    sa.cx = cl->x;
+    sa.cy = cl->y;
+    sa.cz = cl->z;
+    sa.corig = cl->orig;
+    sa.cslot = cl->body_slot;
+    sa.cbounds = cl->bounds;
+    sa.cluster_min = cl->min_verts > 1 ? cl->min_verts : 1;
+  }
   const ogjk::OutArgs oa{d_simplices, d_distances, d_normals, d_f64};
   OGJK_HIP(ogjk::launch_scene(mode, sa, oa, n_pairs, d_workspace, (hipStream_t)stream),
            "kernel launch");
+  return OGJK_OK;
+}
+
+int64_t ogjk_cluster_slots(int64_t n_verts, int32_t n_bodies) {
+  return ogjk::cluster_slots(n_verts, n_bodies);
+}
+
+ogjk_status ogjk_build_clusters(const ogjk_scene* scene, ogjk_clusters* cl, void* stream) {
+  if (!scene || !cl) return fail(OGJK_ERR_INVALID, "NULL scene or clusters");
+  if (scene->n_bodies < 0) return fail(OGJK_ERR_INVALID, "negative body count");
+  if (scene->n_bodies == 0) return OGJK_OK;
+  if (!scene->x || !scene->y || !scene->z || !scene->body_off || !scene->body_cnt)
+    return fail(OGJK_ERR_INVALID, "incomplete scene");
+  if (!cl->x || !cl->y || !cl->z || !cl->orig || !cl->body_slot || !cl->bounds)
+    return fail(OGJK_ERR_INVALID, "incomplete clusters");
+  const hipError_t e = ogjk::build_clusters(
+      scene->x, scene->y, scene->z, scene->body_off, scene->body_cnt, scene->n_bodies, cl->x,
+      cl->y, cl->z, cl->orig, cl->body_slot, cl->bounds, cl->n_slots, (hipStream_t)stream);
+  if (e == hipErrorInvalidValue)
+    return fail(OGJK_ERR_INVALID, "clusters need more than %lld slots, or a body exceeds 2^27 "
+                "vertices", (long long)cl->n_slots);
+  if (e == hipErrorOutOfMemory) return fail(OGJK_ERR_NOMEM, "cluster build scratch");
+  OGJK_HIP(e, "cluster build");
   return OGJK_OK;
 }
 

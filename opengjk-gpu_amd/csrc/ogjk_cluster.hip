@@ -1,0 +1,324 @@
+// Vertex clusters for culled support sweeps: per body, the vertices are
+// copied in Hilbert order of their direction from the body's centre, padded
+// to a multiple of kCluster slots, and every run of kCluster slots gets a
+// bounding sphere and box.  Built once per scene; the support sweep that reads it
+// is in ogjk_core.hpp (support(const CBody&, ...)).
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <vector>
+
+#include "ogjk_core.hpp"
+#include "ogjk_internal.hpp"
+
+namespace ogjk {
+
+namespace {
+
+// Index of (x, y) on the Hilbert curve over a 2^16 x 2^16 grid.
+OGJK_DEV uint32_t hilbert16(uint32_t x, uint32_t y) {
+  constexpr uint32_t n = 1u << 16;
+  uint32_t d = 0;
+  for (uint32_t s = n / 2; s > 0; s >>= 1) {
+    const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+    d += s * s * ((3u * rx) ^ ry);
+    if (ry == 0) {
+      if (rx == 1) {
+        x = n - 1 - x;
+        y = n - 1 - y;
+      }
+      const uint32_t t = x;
+      x = y;
+      y = t;
+    }
+  }
+  return d;
+}
+
+// Octahedral map of the direction v (any length) to [0, 2^16)^2.
+OGJK_DEV void octa16(float vx, float vy, float vz, uint32_t& qx, uint32_t& qy) {
+  const float l1 = fabsf(vx) + fabsf(vy) + fabsf(vz);
+  float u = 0.0f, w = 0.0f;
+  if (l1 > 0.0f) {
+    u = vx / l1;
+    w = vy / l1;
+    if (vz < 0.0f) {
+      const float uu = (1.0f - fabsf(w)) * (u >= 0.0f ? 1.0f : -1.0f);
+      const float ww = (1.0f - fabsf(u)) * (w >= 0.0f ? 1.0f : -1.0f);
+      u = uu;
+      w = ww;
+    }
+  }
+  const float fu = (u + 1.0f) * 32767.5f, fw = (w + 1.0f) * 32767.5f;
+  qx = fu > 0.0f ? (uint32_t)fminf(fu, 65535.0f) : 0u;  // NaN -> 0
+  qy = fw > 0.0f ? (uint32_t)fminf(fw, 65535.0f) : 0u;
+}
+
+OGJK_DEV float block_min(float v, float* red) {
+  for (int m = 32; m >= 1; m >>= 1) v = fminf(v, __shfl_xor(v, m, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return fminf(fminf(red[0], red[1]), fminf(red[2], red[3]));
+}
+
+OGJK_DEV float block_max(float v, float* red) { return -block_min(-v, red); }
+
+// One block per body: sort key of every vertex = Hilbert index of its
+// direction from the centre of the body's bounding box (octahedral map), so
+// that consecutive vertices lie in the same narrow cone of directions, the
+// set a support query in nearby directions reaches; value = body-local index.
+__global__ void __launch_bounds__(256) k_order_keys(const float* __restrict__ x,
+                                                    const float* __restrict__ y,
+                                                    const float* __restrict__ z,
+                                                    const int64_t* __restrict__ off,
+                                                    const int32_t* __restrict__ cnt, int b0,
+                                                    const int* __restrict__ kbeg,
+                                                    uint32_t* __restrict__ keys,
+                                                    int* __restrict__ vals) {
+  __shared__ float red[4];
+  const int b = b0 + blockIdx.x;
+  const int n = cnt[b];
+  if (n <= 0) return;
+  const int64_t o = off[b];
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const float p[3] = {x[o + i], y[o + i], z[o + i]};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      lo[k] = fminf(lo[k], p[k]);
+      hi[k] = fmaxf(hi[k], p[k]);
+    }
+  }
+  float c[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    lo[k] = block_min(lo[k], red);
+    hi[k] = block_max(hi[k], red);
+    c[k] = 0.5f * lo[k] + 0.5f * hi[k];
+  }
+  const int base = kbeg[blockIdx.x];
+  for (int i = threadIdx.x; i < n; i += 256) {
+    uint32_t qx, qy;
+    octa16(x[o + i] - c[0], y[o + i] - c[1], z[o + i] - c[2], qx, qy);
+    keys[base + i] = hilbert16(qx, qy);
+    vals[base + i] = i;
+  }
+}
+
+// One block per body: copy the vertices to their slots in sorted order; the
+// padding slots up to the next multiple of kCluster get orig = -1.
+__global__ void __launch_bounds__(256) k_cluster_scatter(
+    const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
+    const int64_t* __restrict__ off, const int32_t* __restrict__ cnt,
+    const int64_t* __restrict__ slot, int b0, const int* __restrict__ kbeg,
+    const int* __restrict__ sorted, float* __restrict__ cx, float* __restrict__ cy,
+    float* __restrict__ cz, int32_t* __restrict__ orig) {
+  const int b = b0 + blockIdx.x;
+  const int n = cnt[b];
+  if (n <= 0) return;
+  const int64_t o = off[b], s = slot[b];
+  const int base = kbeg[blockIdx.x];
+  const int padded = (n + kCluster - 1) / kCluster * kCluster;
+  for (int i = threadIdx.x; i < padded; i += 256) {
+    if (i < n) {
+      const int l = sorted[base + i];
+      cx[s + i] = x[o + l];
+      cy[s + i] = y[o + l];
+      cz[s + i] = z[o + l];
+      orig[s + i] = l;
+    } else {
+      cx[s + i] = 0.0f;
+      cy[s + i] = 0.0f;
+      cz[s + i] = 0.0f;
+      orig[s + i] = -1;
+    }
+  }
+}
+
+// One kCluster-lane slice per cluster: centre c = midpoint of the bounding
+// box (rounded to float), radius r = max |x - c| and half extents
+// h_k = max |x_k - c_k|, computed in FP64 and rounded up to float.
+__global__ void __launch_bounds__(256) k_cluster_bounds(const float* __restrict__ cx,
+                                                        const float* __restrict__ cy,
+                                                        const float* __restrict__ cz,
+                                                        const int32_t* __restrict__ orig,
+                                                        int64_t n_clusters,
+                                                        ClusterBound* __restrict__ bounds) {
+  const int64_t c = ((int64_t)blockIdx.x * 256 + threadIdx.x) / kCluster;
+  const int t = threadIdx.x & (kCluster - 1);
+  const bool in = c < n_clusters;
+  const int64_t s = c * kCluster + t;
+  const bool valid = in && orig[s] >= 0;
+  const float p[3] = {valid ? cx[s] : 0.0f, valid ? cy[s] : 0.0f, valid ? cz[s] : 0.0f};
+  float lo[3], hi[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    lo[k] = valid ? p[k] : INFINITY;
+    hi[k] = valid ? p[k] : -INFINITY;
+  }
+#pragma unroll
+  for (int m = 1; m < kCluster; m <<= 1) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      lo[k] = fminf(lo[k], __shfl_xor(lo[k], m, kCluster));
+      hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], m, kCluster));
+    }
+  }
+  float ctr[3];
+  double e[3], r = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    ctr[k] = 0.5f * lo[k] + 0.5f * hi[k];
+    e[k] = valid ? fabs((double)p[k] - (double)ctr[k]) : 0.0;
+  }
+  if (valid) r = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+#pragma unroll
+  for (int m = 1; m < kCluster; m <<= 1) {
+    r = fmax(r, __shfl_xor(r, m, kCluster));
+#pragma unroll
+    for (int k = 0; k < 3; k++) e[k] = fmax(e[k], __shfl_xor(e[k], m, kCluster));
+  }
+  if (in && t == 0) {
+    ClusterBound cb;
+    cb.c = make_float4(ctr[0], ctr[1], ctr[2], nextafterf((float)r, INFINITY));
+    cb.h = make_float4(nextafterf((float)e[0], INFINITY), nextafterf((float)e[1], INFINITY),
+                       nextafterf((float)e[2], INFINITY), 0.0f);
+    bounds[c] = cb;
+  }
+}
+
+}  // namespace
+
+int64_t cluster_slots(int64_t n_verts, int32_t n_bodies) {
+  return n_verts + (int64_t)(kCluster - 1) * (n_bodies > 0 ? n_bodies : 0);
+}
+
+// Builds the clusters of every body.  Bodies are processed in batches of at
+// most kBatch vertices so the sort's scratch stays bounded; body_slot is
+// computed on the host from the counts.
+hipError_t build_clusters(const float* x, const float* y, const float* z, const int64_t* off,
+                          const int32_t* cnt, int32_t n_bodies, float* cx, float* cy, float* cz,
+                          int32_t* orig, int64_t* body_slot, float* bounds, int64_t n_slots,
+                          hipStream_t st) {
+  constexpr int64_t kBatch = int64_t(1) << 27;
+  if (n_bodies <= 0) return hipSuccess;
+  std::vector<int32_t> hcnt(n_bodies);
+  hipError_t e = hipMemcpyAsync(hcnt.data(), cnt, sizeof(int32_t) * n_bodies,
+                                hipMemcpyDeviceToHost, st);
+  if (e != hipSuccess || (e = hipStreamSynchronize(st)) != hipSuccess) return e;
+  std::vector<int64_t> hslot(n_bodies);
+  int64_t total = 0, maxn = 0;
+  for (int b = 0; b < n_bodies; b++) {
+    const int64_t n = hcnt[b] > 0 ? hcnt[b] : 0;
+    hslot[b] = total;
+    total += (n + kCluster - 1) / kCluster * kCluster;
+    maxn = std::max(maxn, n);
+  }
+  if (total > n_slots || maxn > kBatch) return hipErrorInvalidValue;
+  e = hipMemcpyAsync(body_slot, hslot.data(), sizeof(int64_t) * n_bodies, hipMemcpyHostToDevice,
+                     st);
+  if (e != hipSuccess) return e;
+
+  // batches of consecutive bodies
+  std::vector<int> bstart{0};
+  std::vector<int> kbeg_h;  // batch-local key offset of every body, batch after batch
+  std::vector<int64_t> bverts;
+  {
+    int64_t acc = 0;
+    for (int b = 0; b < n_bodies; b++) {
+      const int64_t n = hcnt[b] > 0 ? hcnt[b] : 0;
+      if (acc + n > kBatch) {
+        bstart.push_back(b);
+        bverts.push_back(acc);
+        acc = 0;
+      }
+      kbeg_h.push_back((int)acc);
+      acc += n;
+    }
+    bverts.push_back(acc);
+    bstart.push_back(n_bodies);
+  }
+  int64_t maxv = 0;
+  int maxb = 0;
+  for (size_t i = 0; i + 1 < bstart.size(); i++) {
+    maxv = std::max(maxv, bverts[i]);
+    maxb = std::max(maxb, bstart[i + 1] - bstart[i]);
+  }
+  maxv = std::max<int64_t>(maxv, 1);
+
+  uint32_t *keys = nullptr, *keys2 = nullptr;
+  int *vals = nullptr, *vals2 = nullptr, *kbeg = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+  auto cleanup = [&]() {
+    hipFree(keys);
+    hipFree(keys2);
+    hipFree(vals);
+    hipFree(vals2);
+    hipFree(kbeg);
+    hipFree(tmp);
+  };
+  if ((e = hipMalloc(&keys, 4 * maxv)) != hipSuccess || (e = hipMalloc(&keys2, 4 * maxv)) ||
+      (e = hipMalloc(&vals, 4 * maxv)) || (e = hipMalloc(&vals2, 4 * maxv)) ||
+      (e = hipMalloc(&kbeg, sizeof(int) * ((size_t)maxb + 1)))) {
+    cleanup();
+    return e;
+  }
+  for (size_t i = 0; i + 1 < bstart.size(); i++) {
+    const int b0 = bstart[i], nb = bstart[i + 1] - b0;
+    const int nv = (int)bverts[i];
+    if (nb == 0) continue;
+    // segment offsets: kbeg[0..nb], kbeg[nb] = nv (end of the last body)
+    std::vector<int> seg(kbeg_h.begin() + b0, kbeg_h.begin() + b0 + nb);
+    seg.push_back(nv);
+    if ((e = hipMemcpyAsync(kbeg, seg.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice,
+                            st)) != hipSuccess) {
+      cleanup();
+      return e;
+    }
+    hipLaunchKernelGGL(k_order_keys, dim3(nb), dim3(256), 0, st, x, y, z, off, cnt, b0, kbeg, keys,
+                       vals);
+    size_t need = 0;
+    e = hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, need, keys, keys2, vals, vals2, nv,
+                                                    nb, kbeg, kbeg + 1, 0, 32, st);
+    if (e != hipSuccess) {
+      cleanup();
+      return e;
+    }
+    if (need > tmp_bytes) {
+      hipFree(tmp);
+      tmp = nullptr;
+      if ((e = hipMalloc(&tmp, need)) != hipSuccess) {
+        cleanup();
+        return e;
+      }
+      tmp_bytes = need;
+    }
+    e = hipcub::DeviceSegmentedRadixSort::SortPairs(tmp, need, keys, keys2, vals, vals2, nv, nb,
+                                                    kbeg, kbeg + 1, 0, 32, st);
+    if (e != hipSuccess) {
+      cleanup();
+      return e;
+    }
+    hipLaunchKernelGGL(k_cluster_scatter, dim3(nb), dim3(256), 0, st, x, y, z, off, cnt, body_slot,
+                       b0, kbeg, vals2, cx, cy, cz, orig);
+    // the host vectors of this batch die at the end of the iteration
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) {
+      cleanup();
+      return e;
+    }
+  }
+  const int64_t n_clusters = total / kCluster;
+  if (n_clusters > 0) {
+    const int64_t threads = n_clusters * kCluster;
+    hipLaunchKernelGGL(k_cluster_bounds, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st,
+                       cx, cy, cz, orig, n_clusters, (ClusterBound*)bounds);
+  }
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  cleanup();
+  return e;
+}
+
+}  // namespace ogjk

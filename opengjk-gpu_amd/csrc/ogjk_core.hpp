@@ -28,6 +28,7 @@ namespace ogjk {
 // T is the storage type: float (the product path) or double (plain FP64).
 template <class T, int STRIDE>
 struct Body {
+  using Elem = T;
   const T* x;
   const T* y;
   const T* z;
@@ -156,7 +157,9 @@ OGJK_DEV Cand candidates32(const Body<float, STRIDE>& b, const Sweep32& s, float
 // Group argmax of the lanes' candidates (lowest index on ties), then the
 // acceptance rule of the query and the winner's point.  Usual case: one lane
 // holds the only candidate and a single parallel shuffle fetches it.
-template <int LPP>
+// STRIDED: lane l swept indices l, l+LPP, ..., so the winner's index names
+// its lane; otherwise the lane holding the winning index is found by ballot.
+template <int LPP, bool STRIDED = true>
 OGJK_DEV Sup finish1(Cand c, const Query& q) {
   const uint64_t holders = group_ballot<LPP>(c.v > -INFINITY);
   int src;
@@ -165,8 +168,13 @@ OGJK_DEV Sup finish1(Cand c, const Query& q) {
     c.v = shfl_d<LPP>(c.v, src);
     c.i = shfl_i<LPP>(c.i, src);
   } else {
+    const int mine = c.i;
     group_argmax_d<LPP>(c.v, c.i);
-    src = c.i & (LPP - 1);  // lane l sweeps indices l, l+LPP, ...
+    if constexpr (STRIDED) {
+      src = c.i & (LPP - 1);
+    } else {
+      src = (int)__builtin_ctzll(group_ballot<LPP>(mine == c.i) | (1ull << 63));
+    }
   }
   if (q.has_inc && !(c.v > dot(q.inc_pt, q.dir))) return Sup{q.inc_idx, q.inc_pt};
   if (!q.has_inc && !(c.v > -1e10)) return Sup{-1, D3{0.0, 0.0, 0.0}};
@@ -185,15 +193,198 @@ OGJK_DEV Sup support(const Body<float, STRIDE>& A, const Query& q, int lane) {
   return finish1<LPP>(candidates32<LPP>(A, sw, t, multi, q.dir, lane), q);
 }
 
-// The support points of both bodies, one query after the other.  (Sweeping
-// the two bodies in one loop keeps two sweep states live and costs more in
-// registers than it gains in overlap.)
-template <int LPP, int STRIDE>
-OGJK_DEV void support_pair(const Body<float, STRIDE>& A, const Query& qa,
-                           const Body<float, STRIDE>& B, const Query& qb, int lane, Sup& ra,
-                           Sup& rb) {
-  ra = support<LPP>(A, qa, lane);
-  rb = support<LPP>(B, qb, lane);
+// ------------------------------------------------ clustered support sweep
+//
+// A scene body may carry a second copy of its vertices ordered by direction
+// from the body's centre (Hilbert order of the octahedral map), cut into
+// clusters of kCluster slots, each with a bounding sphere (float centre c,
+// radius r >= max |x - c|) and box (half extents h about c).  A support
+// query then
+//   1. bounds every cluster: ub = c.d + min(r|d|, h.|d|) >= max of x.d,
+//   2. sweeps the cluster with the largest ub: m0 = its FP32 maximum,
+//   3. sweeps only the clusters with ub >= m0 - 2^-15 r1.
+// Rounding: |fl(ub) - ub| <= 2^-17 r1 and |D32 - D| <= 2^-20 r1 (d scaled to
+// max-norm in [0.5,1), |c|_1 <= 3 r1, |h|_1 <= 3 r1, r <= 2 r1), so every skipped vertex
+// lies more than 2^-16 r1 below the maximum: it can neither be the FP64
+// argmax nor tie with it, and the selection is the full sweep's.  Indices
+// reported are the body-local input indices (the copy stores them).
+constexpr int kCluster = 32;
+
+// Bounds of one cluster: centre c and radius r (c.w) of a sphere, and half
+// extents h of the box around c, all rounded outwards.
+struct ClusterBound {
+  float4 c;
+  float4 h;
+};
+
+struct ClusterView {
+  const float* x;  // cluster-ordered vertices of every body
+  const float* y;
+  const float* z;
+  const int32_t* orig;         // body-local input index of each slot
+  const ClusterBound* bounds;  // per cluster
+};
+
+struct CBody : Body<float, 1> {
+  ClusterView cv;
+  int64_t slot;  // first slot of the body (a multiple of kCluster); < 0: no culling
+};
+
+// c.d + min(r |d|, h.|d|) >= max over the cluster of x.d
+OGJK_DEV float cluster_ub(const ClusterBound& b, float dx, float dy, float dz, float dn) {
+  const float cd = fmaf(b.c.x, dx, fmaf(b.c.y, dy, b.c.z * dz));
+  const float box = fmaf(b.h.x, fabsf(dx), fmaf(b.h.y, fabsf(dy), b.h.z * fabsf(dz)));
+  return cd + fminf(b.c.w * dn, box);
+}
+
+// Calls f(k) for every body-local slot k of the clusters whose bound reaches
+// L, each slot on exactly one lane, in the same order on every call.
+template <int LPP, class F>
+OGJK_DEV void visit_clusters(const CBody& A, const ClusterBound* cb, int nc, float dx, float dy,
+                             float dz, float dn, float L, int lane, F&& f) {
+  for (int base = 0; base < nc; base += LPP) {
+    const int c = base + lane;
+    const bool live = c < nc && !(cluster_ub(cb[c], dx, dy, dz, dn) < L);
+    uint64_t m = group_ballot<LPP>(live);
+    while (m != 0) {
+      if constexpr (LPP >= kCluster) {
+        // LPP/kCluster clusters per round, one per kCluster-lane slice
+        constexpr int PER = LPP / kCluster;
+        int cid = -1;
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+          if (m != 0) {
+            if (j == lane / kCluster) cid = (int)__builtin_ctzll(m);
+            m &= m - 1;
+          }
+        }
+        if (cid >= 0) {
+          const int k = (base + cid) * kCluster + (lane & (kCluster - 1));
+          if (k < A.n) f(k);
+        }
+      } else {
+        const int cid = (int)__builtin_ctzll(m);
+        m &= m - 1;
+#pragma unroll
+        for (int t = lane; t < kCluster; t += LPP) {
+          const int k = (base + cid) * kCluster + t;
+          if (k < A.n) f(k);
+        }
+      }
+    }
+  }
+}
+
+template <int LPP>
+OGJK_DEV Sup support(const CBody& A, const Query& q, int lane) {
+  if (A.slot < 0) return support<LPP>(static_cast<const Body<float, 1>&>(A), q, lane);
+  Sweep32 sw = sweep32_begin(q.dir);
+  const float dx = sw.dx, dy = sw.dy, dz = sw.dz;
+  const float dn = sqrtf(fmaf(dx, dx, fmaf(dy, dy, dz * dz))) * (1.0f + 0x1.0p-20f);
+  const int nc = (A.n + kCluster - 1) / kCluster;
+  const ClusterBound* cb = A.cv.bounds + A.slot / kCluster;
+  const float* sx = A.cv.x + A.slot;
+  const float* sy = A.cv.y + A.slot;
+  const float* sz = A.cv.z + A.slot;
+
+  // 1-2: the cluster with the largest bound gives a lower bound of the max
+  float bub = -INFINITY;
+  int bc = 0;
+  for (int c = lane; c < nc; c += LPP) {
+    const float ub = cluster_ub(cb[c], dx, dy, dz, dn);
+    if (ub > bub) {
+      bub = ub;
+      bc = c;
+    }
+  }
+  const float gub = group_max_f<LPP>(bub);
+  bc = shfl_i<LPP>(bc, (int)__builtin_ctzll(group_ballot<LPP>(bub == gub) | (1ull << 63)));
+  float m0 = -INFINITY;
+#pragma unroll
+  for (int t = lane; t < kCluster; t += LPP) {
+    const int k = bc * kCluster + t;
+    if (k < A.n) m0 = fmaxf(m0, fmaf(sx[k], dx, fmaf(sy[k], dy, sz[k] * dz)));
+  }
+  const float L = group_max_f<LPP>(m0) - 0x1.0p-15f * A.r1;
+
+  // 3: sweep the surviving clusters
+  visit_clusters<LPP>(A, cb, nc, dx, dy, dz, dn, L, lane, [&](int k) {
+    const float px = sx[k], py = sy[k], pz = sz[k];
+    const float d = fmaf(px, dx, fmaf(py, dy, pz * dz));
+    sw.second = fmaxf(sw.second, fminf(sw.best, d));
+    if (d > sw.best) {
+      sw.best = d;
+      sw.bi = k;
+      sw.bx = px;
+      sw.by = py;
+      sw.bz = pz;
+    }
+  });
+  const float M = group_max_f<LPP>(sw.best);
+  const float T = M - 0x1.0p-19f * A.r1;
+  const bool multi = group_any<LPP>(sw.second >= T);
+
+  // FP64 re-rank of the window; ties go to the lowest input index
+  const int32_t* orig = A.cv.orig + A.slot;
+  Cand c{-INFINITY, 0x7fffffff, D3{0.0, 0.0, 0.0}};
+  if (!multi) {
+    if (sw.bi >= 0 && sw.best >= T) {
+      c.p = D3{(double)sw.bx, (double)sw.by, (double)sw.bz};
+      c.v = dot(c.p, q.dir);
+      c.i = orig[sw.bi];
+    }
+  } else {
+    visit_clusters<LPP>(A, cb, nc, dx, dy, dz, dn, L, lane, [&](int k) {
+      const float px = sx[k], py = sy[k], pz = sz[k];
+      const float d = fmaf(px, dx, fmaf(py, dy, pz * dz));
+      if (d >= T) {
+        const D3 p{(double)px, (double)py, (double)pz};
+        const double v = dot(p, q.dir);
+        const int oi = orig[k];
+        if (v > c.v || (v == c.v && oi < c.i)) {
+          c.v = v;
+          c.i = oi;
+          c.p = p;
+        }
+      }
+    });
+  }
+  if (!(c.v == c.v)) c.v = -INFINITY;
+  return finish1<LPP, false>(c, q);
+}
+
+// Field-wise value selects (a select between two structs' addresses would
+// put them in scratch memory).
+template <class T, int STRIDE>
+OGJK_DEV Body<T, STRIDE> pick(bool c, const Body<T, STRIDE>& a, const Body<T, STRIDE>& b) {
+  return Body<T, STRIDE>{c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z, c ? a.n : b.n,
+                         c ? a.r1 : b.r1};
+}
+OGJK_DEV CBody pick(bool c, const CBody& a, const CBody& b) {
+  return CBody{pick(c, static_cast<const Body<float, 1>&>(a), static_cast<const Body<float, 1>&>(b)),
+               a.cv, c ? a.slot : b.slot};
+}
+OGJK_DEV Query pick(bool c, const Query& a, const Query& b) {
+  return Query{sel(c, a.dir, b.dir), c ? a.has_inc : b.has_inc, c ? a.inc_idx : b.inc_idx,
+               sel(c, a.inc_pt, b.inc_pt)};
+}
+
+// The support points of both bodies, one query after the other through one
+// copy of the sweep code (two sweeps in one loop keep two sweep states live;
+// two inlined copies double the code the register allocator sees).
+template <int LPP, class BT>
+OGJK_DEV void support_pair(const BT& A, const Query& qa, const BT& B, const Query& qb, int lane,
+                           Sup& ra, Sup& rb) {
+#pragma clang loop unroll(disable)
+  for (int w = 0; w < 2; w++) {
+    const bool second = w != 0;
+    const Sup r = support<LPP>(pick(second, B, A), pick(second, qb, qa), lane);
+    if (second) {
+      rb = r;
+    } else {
+      ra = r;
+    }
+  }
 }
 
 // Plain FP64 path over double storage: the same selection rules computed
@@ -740,8 +931,8 @@ OGJK_DEV void witnesses(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, Simp
 // ------------------------------------------------------------- GJK driver
 
 // compute_minimum_distance: GJK/cpu/openGJK.c:952-1056.  Returns ||v||.
-template <int LPP, class T, int STRIDE>
-OGJK_DEV double gjk(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, Simplex& s, Wit& w, int lane) {
+template <int LPP, class BT>
+OGJK_DEV double gjk(const BT& A, const BT& B, Simplex& s, Wit& w, int lane) {
   const double eps_rel = kEps64 * 1e4;  // (gkFloat)DBL_EPSILON * 1e4f
   const double eps_tot = kEps64 * 1e2;
   const double eps_rel2 = eps_rel * eps_rel;

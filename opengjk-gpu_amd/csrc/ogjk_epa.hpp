@@ -195,9 +195,8 @@ OGJK_DEV void fetch_face(const Faces<FPL>& F, int slot, D3& n, int& verts) {
 
 // support_epa (GJK/cpu/EPA.c:307-344): farthest vertex of A along dir and of
 // B along -dir; returns the Minkowski point A[ia] - B[ib].
-template <int LPP, class T, int STRIDE>
-OGJK_DEV D3 support_epa(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, D3 dir, int& ia,
-                        int& ib, int lane) {
+template <int LPP, class BT>
+OGJK_DEV D3 support_epa(const BT& A, const BT& B, D3 dir, int& ia, int& ib, int lane) {
   const D3 zero{0.0, 0.0, 0.0};
   Sup a, b;
   support_pair<LPP>(A, Query{dir, false, 0, zero}, B, Query{neg(dir), false, 0, zero}, lane, a,
@@ -250,9 +249,9 @@ OGJK_DEV void face_result(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, co
 }
 
 // computeCollisionInformation for one pair (GJK/cpu/EPA.c:362-864).
-template <int LPP, class T, int STRIDE>
-OGJK_DEV void epa(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, Simplex& s, double& distance,
-                  Wit& w, D3& normal, EpaShared& sh, int lane) {
+template <int LPP, class BT>
+OGJK_DEV void epa(const BT& A, const BT& B, Simplex& s, double& distance, Wit& w, D3& normal,
+                  EpaShared& sh, int lane) {
   constexpr int FPL = kEpaMaxFaces / LPP;
   static_assert(FPL * LPP == kEpaMaxFaces, "LPP must divide 128");
 

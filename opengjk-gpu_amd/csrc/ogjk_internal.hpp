@@ -19,6 +19,15 @@ struct SceneArgs {       // SoA float scene (or double for the plain path)
   const float* r1;       // may be null for the double scene
   int32_t n_bodies;
   const int32_t* pairs;  // 2 per pair, or null: pair i = (2i, 2i+1)
+  // optional vertex clusters (float scene only): bodies with at least
+  // cluster_min vertices use the culled support sweep
+  const float* cx = nullptr;
+  const float* cy = nullptr;
+  const float* cz = nullptr;
+  const int32_t* corig = nullptr;
+  const int64_t* cslot = nullptr;
+  const float* cbounds = nullptr;
+  int32_t cluster_min = 0x7fffffff;
 };
 
 struct PolyArgs {                // gkPolytope descriptors, AoS coordinates
@@ -71,6 +80,12 @@ hipError_t launch_scene_f64(int mode, const SceneArgs& s, const OutArgs& o, int 
                             hipStream_t st);
 hipError_t launch_poly(int mode, const PolyArgs& p, const OutArgs& o, int n, void* ws,
                        hipStream_t st);
+// Vertex clusters (see ogjk_cluster.hip): slots needed, and the build.
+int64_t cluster_slots(int64_t n_verts, int32_t n_bodies);
+hipError_t build_clusters(const float* x, const float* y, const float* z, const int64_t* off,
+                          const int32_t* cnt, int32_t n_bodies, float* cx, float* cy, float* cz,
+                          int32_t* orig, int64_t* body_slot, float* bounds, int64_t n_slots,
+                          hipStream_t st);
 hipError_t launch_pack_f64(const float* aos, int64_t n_verts, double* x, double* y, double* z,
                            hipStream_t st);
 hipError_t launch_pack(const float* aos, const int64_t* off, const int32_t* cnt, int n_bodies,

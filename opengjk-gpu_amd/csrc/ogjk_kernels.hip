@@ -1,6 +1,8 @@
 // opengjk-amd kernels for gfx950: one group of LPP lanes per pair, block of
 // 256 threads.  GJK and EPA run fused in one kernel when both are asked for,
 // so a pair's vertices are swept from L1/L2 by the same wave throughout.
+#include <type_traits>
+
 #include "ogjk_epa.hpp"
 #include "ogjk_internal.hpp"
 
@@ -17,7 +19,9 @@ namespace ogjk {
 
 // ----------------------------------------------------------------- sources
 
-template <class T>
+// CL: bodies carry their vertex clusters (float scenes built with clusters);
+// without, the kernels contain no culled sweep at all.
+template <class T, bool CL = false>
 struct SceneSrc {
   const T* x;
   const T* y;
@@ -27,7 +31,10 @@ struct SceneSrc {
   const float* r1;
   int32_t nb;
   const int32_t* pairs;
-  using BodyT = Body<T, 1>;
+  ClusterView cv;        // float scene: vertex clusters, or null bounds
+  const int64_t* cslot;  // first cluster slot of each body
+  int32_t cluster_min;   // bodies with at least this many vertices are culled
+  using BodyT = std::conditional_t<CL, CBody, Body<T, 1>>;
 
   // max(nA, nB) of pair i, or 0 for an invalid pair (it goes to bucket 0)
   OGJK_DEV int pair_size(int i) const {
@@ -48,8 +55,15 @@ struct SceneSrc {
     }
     if (a < 0 || a >= nb || b < 0 || b >= nb) return false;
     const int64_t oa = off[a], ob = off[b];
-    A = BodyT{x + oa, y + oa, z + oa, cnt[a], r1 ? r1[a] : 0.0f};
-    B = BodyT{x + ob, y + ob, z + ob, cnt[b], r1 ? r1[b] : 0.0f};
+    const int na = cnt[a], nb_ = cnt[b];
+    const float ra = r1 ? r1[a] : 0.0f, rb = r1 ? r1[b] : 0.0f;
+    if constexpr (CL) {
+      A = CBody{{x + oa, y + oa, z + oa, na, ra}, cv, na >= cluster_min ? cslot[a] : int64_t(-1)};
+      B = CBody{{x + ob, y + ob, z + ob, nb_, rb}, cv, nb_ >= cluster_min ? cslot[b] : int64_t(-1)};
+    } else {
+      A = BodyT{x + oa, y + oa, z + oa, na, ra};
+      B = BodyT{x + ob, y + ob, z + ob, nb_, rb};
+    }
     return true;
   }
 };
@@ -619,15 +633,23 @@ static hipError_t launch_any(int mode, const Src& src, const OutArgs& o, int n, 
 
 hipError_t launch_scene(int mode, const SceneArgs& s, const OutArgs& o, int n, void* ws,
                         hipStream_t st) {
+  const ClusterView cv{s.cx, s.cy, s.cz, s.corig, (const ClusterBound*)s.cbounds};
+  if (s.cbounds) {
+    const SceneSrc<float, true> src{(const float*)s.x, (const float*)s.y, (const float*)s.z,
+                                    s.off, s.cnt, s.r1, s.n_bodies, s.pairs, cv, s.cslot,
+                                    s.cluster_min};
+    return launch_any(mode, src, o, n, ws, st);
+  }
   const SceneSrc<float> src{(const float*)s.x, (const float*)s.y, (const float*)s.z, s.off,
-                            s.cnt, s.r1, s.n_bodies, s.pairs};
+                            s.cnt, s.r1, s.n_bodies, s.pairs, cv, nullptr, 0x7fffffff};
   return launch_any(mode, src, o, n, ws, st);
 }
 
 hipError_t launch_scene_f64(int mode, const SceneArgs& s, const OutArgs& o, int n, void* ws,
                             hipStream_t st) {
   const SceneSrc<double> src{(const double*)s.x, (const double*)s.y, (const double*)s.z, s.off,
-                             s.cnt, nullptr, s.n_bodies, s.pairs};
+                             s.cnt, nullptr, s.n_bodies, s.pairs, ClusterView{}, nullptr,
+                             0x7fffffff};
   return launch_any(mode, src, o, n, ws, st);
 }
 

@@ -38,7 +38,7 @@ C_ABI_SYMBOLS = [
     "ogjk_set_config", "ogjk_set_buckets", "ogjk_workspace_bytes", "ogjk_collide_scene_ws", "ogjk_pack_scene",
     "ogjk_collide_scene", "ogjk_pack_scene_f64", "ogjk_collide_scene_f64",
     "ogjk_collide_polytopes", "ogjk_collide_polytopes_indexed", "ogjk_collide_host",
-    "ogjk_collide_host_indexed",
+    "ogjk_collide_host_indexed", "ogjk_cluster_slots", "ogjk_build_clusters",
 ]
 
 
@@ -46,9 +46,16 @@ class OgjkError(RuntimeError):
     pass
 
 
+class _Clusters(C.Structure):
+    _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("orig", C.c_void_p),
+                ("body_slot", C.c_void_p), ("bounds", C.c_void_p), ("n_slots", C.c_int64),
+                ("min_verts", C.c_int32)]
+
+
 class _Scene(C.Structure):
     _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("body_off", C.c_void_p),
-                ("body_cnt", C.c_void_p), ("body_r1", C.c_void_p), ("n_bodies", C.c_int32)]
+                ("body_cnt", C.c_void_p), ("body_r1", C.c_void_p), ("n_bodies", C.c_int32),
+                ("clusters", C.POINTER(_Clusters))]
 
 
 class _Scene64(C.Structure):
@@ -79,6 +86,8 @@ def lib():
                                                 vp, C.c_size_t, vp]),
             "ogjk_pack_scene": (C.c_int, [vp, vp, vp, i32, i64, vp, vp, vp, vp, vp]),
             "ogjk_pack_scene_f64": (C.c_int, [vp, i64, vp, vp, vp, vp]),
+            "ogjk_cluster_slots": (i64, [i64, i32]),
+            "ogjk_build_clusters": (C.c_int, [C.POINTER(_Scene), C.POINTER(_Clusters), vp]),
             "ogjk_collide_scene": (C.c_int, [C.c_int, C.POINTER(_Scene), vp, i32, vp, vp, vp, vp, vp]),
             "ogjk_collide_scene_f64": (C.c_int, [C.c_int, C.POINTER(_Scene64), vp, i32, vp, vp, vp, vp, vp]),
             "ogjk_collide_polytopes": (C.c_int, [C.c_int, vp, vp, i32, vp, vp, vp, vp]),
@@ -164,7 +173,8 @@ def collide_host_indexed(mode: int, coords, off, cnt, pairs, simplices=None, dis
 class DeviceScene:
     """A batch resident in device memory as an SoA scene (torch tensors)."""
 
-    def __init__(self, batch, device="cuda", f64: bool = False, stream=None):
+    def __init__(self, batch, device="cuda", f64: bool = False, stream=None,
+                 clusters: bool = False, cluster_min: int = 512):
         import torch
         self.torch = torch
         self.n_bodies = len(batch.cnt)
@@ -195,7 +205,40 @@ class DeviceScene:
         del aos
         self.scene = _Scene(self.x.data_ptr(), self.y.data_ptr(), self.z.data_ptr(),
                             self.off.data_ptr(), self.cnt.data_ptr(), self.r1.data_ptr(),
-                            self.n_bodies)
+                            self.n_bodies, None)
+        self.clusters = None
+        if clusters:
+            self.build_clusters(cluster_min, stream)
+
+    def build_clusters(self, min_verts: int = 512, stream=None):
+        """Build the vertex clusters (ogjk_build_clusters) and use them for
+        bodies of at least min_verts vertices."""
+        t = self.torch
+        dev = self.x.device
+        slots = int(lib().ogjk_cluster_slots(self.n_verts, self.n_bodies))
+        ns = max(slots, 32)
+        self.cx = t.empty(ns, dtype=t.float32, device=dev)
+        self.cy = t.empty(ns, dtype=t.float32, device=dev)
+        self.cz = t.empty(ns, dtype=t.float32, device=dev)
+        self.corig = t.empty(ns, dtype=t.int32, device=dev)
+        self.cslot = t.empty(max(self.n_bodies, 1), dtype=t.int64, device=dev)
+        self.cbounds = t.empty((ns // 32 + 1) * 8, dtype=t.float32, device=dev)
+        self.clusters = _Clusters(self.cx.data_ptr(), self.cy.data_ptr(), self.cz.data_ptr(),
+                                  self.corig.data_ptr(), self.cslot.data_ptr(),
+                                  self.cbounds.data_ptr(), ns, int(min_verts))
+        _check(lib().ogjk_build_clusters(C.byref(self.scene), C.byref(self.clusters),
+                                         self._stream(stream)))
+        self.scene.clusters = C.pointer(self.clusters)
+
+    def set_cluster_min(self, min_verts: int) -> None:
+        """Culling threshold in vertices (needs build_clusters)."""
+        if self.clusters is None:
+            raise OgjkError("no clusters built")
+        self.clusters.min_verts = int(min_verts)
+
+    def drop_clusters(self) -> None:
+        """Sweep every body without culling."""
+        self.scene.clusters = None
 
     def _stream(self, stream):
         if stream is None:

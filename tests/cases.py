@@ -103,3 +103,23 @@ def degenerate_batch(n: int = 4000, seed: int = 5):
             A.append(rng.normal(size=(20, 3)).astype(np.float32) * s)
             B.append(rng.normal(size=(20, 3)).astype(np.float32) * s + s * rng.normal(size=3).astype(np.float32))
     return gen.from_bodies(A, B)
+
+
+def lattice_batch(n: int = 200, seed: int = 7):
+    """Large tie-heavy bodies: integer lattice points on cube surfaces (k^3
+    grids, k in [6, 20], up to 2,168 vertices, shuffled), so support queries
+    along axes and diagonals tie across many vertices and many clusters."""
+    rng = np.random.default_rng(seed)
+    A, B = [], []
+    for _ in range(n):
+        bodies = []
+        for _ in range(2):
+            k = int(rng.integers(6, 21))
+            g = np.arange(k, dtype=np.float32) - np.float32((k - 1) / 2)
+            pts = np.array([[x, y, z] for x in g for y in g for z in g
+                            if max(abs(x), abs(y), abs(z)) == g[-1]], np.float32)
+            pts = pts[rng.permutation(len(pts))]
+            bodies.append(pts + rng.integers(-k, k + 1, size=3).astype(np.float32))
+        A.append(bodies[0])
+        B.append(bodies[1])
+    return gen.from_bodies(A, B)

@@ -21,8 +21,10 @@ def torch_cuda():
     return torch
 
 
-def run_scene(torch, batch, mode, plain=False):
+def run_scene(torch, batch, mode, plain=False, cluster_min=None):
     sc = ogjk.DeviceScene(batch, f64=plain)
+    if cluster_min is not None:
+        sc.build_clusters(cluster_min)
     out = sc.collide(mode, batch.n_pairs, f64_record=True, plain_f64=plain)
     torch.cuda.synchronize()
     rec = out["f64"].cpu().numpy()
@@ -56,11 +58,58 @@ def test_workload_parity(torch_cuda, name, mode):
 
 
 @pytest.mark.parametrize("mode", [oracle.GJK, oracle.GJK_EPA])
-def test_degenerate_parity(torch_cuda, mode):
-    b = cases.degenerate_batch(4000)
+@pytest.mark.parametrize("batch", ["degenerate", "lattice"])
+def test_degenerate_parity(torch_cuda, mode, batch):
+    b = cases.degenerate_batch(4000) if batch == "degenerate" else cases.lattice_batch(200)
     ref = oracle_run(mode, b)
     rec, simp, d32, n32 = run_scene(torch_cuda, b, mode)
-    check(compare_f64(rec, simp, ref, mode), f"degenerate mode{mode}")
+    check(compare_f64(rec, simp, ref, mode), f"{batch} mode{mode}")
+
+
+# Cluster-culled support sweeps select exactly what the full sweep selects:
+# min_verts 1 culls every body, including the tie-heavy degenerate cases.
+@pytest.mark.parametrize("name,cmin", [("c1_sphere64_box1.3", 1), ("c2_ellipsoid_4_4096", 1),
+                                       ("c2_ellipsoid_4_4096", 512), ("c3_noisy_sphere_2000", 1)])
+def test_cluster_parity(torch_cuda, name, cmin):
+    b = gen.make(**WORKLOADS[name])
+    ref = oracle_run(oracle.GJK_EPA, b)
+    rec, simp, d32, n32 = run_scene(torch_cuda, b, oracle.GJK_EPA, cluster_min=cmin)
+    check(compare_f64(rec, simp, ref, oracle.GJK_EPA), f"{name} clusters>={cmin}")
+
+
+@pytest.mark.parametrize("mode", [oracle.GJK, oracle.GJK_EPA])
+@pytest.mark.parametrize("batch", ["degenerate", "lattice"])
+def test_cluster_degenerate_parity(torch_cuda, mode, batch):
+    b = cases.degenerate_batch(4000) if batch == "degenerate" else cases.lattice_batch(200)
+    ref = oracle_run(mode, b)
+    rec, simp, _, _ = run_scene(torch_cuda, b, mode, cluster_min=1)
+    check(compare_f64(rec, simp, ref, mode), f"{batch} clusters mode{mode}")
+
+
+def test_cluster_layout(torch_cuda):
+    """Every body's slots hold a permutation of its vertices, padded with -1,
+    and every cluster's sphere contains its vertices."""
+    b = gen.make(gen.ELLIPSOID, 200, seed=5, box=1.5, nmin=4, nmax=700)
+    sc = ogjk.DeviceScene(b)
+    sc.build_clusters(1)
+    torch_cuda.cuda.synchronize()
+    slot = sc.cslot.cpu().numpy()
+    orig = sc.corig.cpu().numpy()
+    cx, cy, cz = (t.cpu().numpy().astype(np.float64) for t in (sc.cx, sc.cy, sc.cz))
+    bounds = sc.cbounds.cpu().numpy().reshape(-1, 8).astype(np.float64)
+    for i, (o, n) in enumerate(zip(b.off, b.cnt)):
+        s0 = slot[i]
+        assert s0 % 32 == 0
+        pad = -(-n // 32) * 32
+        idx = orig[s0:s0 + pad]
+        assert sorted(idx[:n].tolist()) == list(range(n)) and (idx[n:] == -1).all()
+        pts = b.coords[o + idx[:n]].astype(np.float64)
+        assert np.array_equal(np.stack([cx[s0:s0 + n], cy[s0:s0 + n], cz[s0:s0 + n]], 1), pts)
+        for c in range(pad // 32):
+            q = pts[c * 32:(c + 1) * 32]
+            ctr, r, h = bounds[s0 // 32 + c, :3], bounds[s0 // 32 + c, 3], bounds[s0 // 32 + c, 4:7]
+            assert (np.linalg.norm(q - ctr, axis=1) <= r).all()
+            assert (np.abs(q - ctr) <= h).all()
 
 
 def test_plain_f64_equals_mixed(torch_cuda):
