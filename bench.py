@@ -1,10 +1,18 @@
 #!/usr/bin/env python3
 """Benchmark: batched GJK+EPA pairs/s on one MI355X.
 
-Workload (default): 100,000 pairs of 64-vertex polytopes, points uniform on
-spheres of radius U[0.5,1.5] with centres in U[-1.29,1.29]^3 (about 60% of
-pairs penetrate), GJK then EPA for every pair, FP32 vertex storage with the
-FP32 support sweep + exact FP64 re-rank, FP64 sub-distance/EPA arithmetic.
+Workloads (--workload):
+  sphere64 (default): 100,000 pairs of 64-vertex polytopes, points uniform on
+      spheres of radius U[0.5,1.5], centres U[-1.29,1.29]^3 (about 60% of pairs
+      penetrate).
+  ellipsoid_uneven: 1,000,000 pairs, vertex counts log-uniform in [4, 4096],
+      points on rotated ellipsoids (axes U[0.3,2]), centres U[-1.5,1.5]^3.
+  noisy_sphere_large: 10,000 pairs of 100,000-vertex polytopes, unit-sphere
+      points perturbed by N(0, 0.01^2) per coordinate, centres U[-1.5,1.5]^3
+      (about 50% penetrate).
+Every pair runs GJK then EPA: FP32 vertex storage, FP32 support sweep with an
+exact FP64 re-rank, FP64 sub-distance and EPA arithmetic; the larger
+workloads sweep large bodies through their vertex clusters.
 
 A step is one GJK+EPA pass over the whole batch (GJK kernel, device worklist
 of penetrating pairs, persistent EPA kernel), inputs resident in HBM as an
@@ -26,24 +34,39 @@ sys.path[:0] = [ROOT, os.path.join(ROOT, "opengjk-gpu_amd")]
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+EPS64 = 2.220446049250313e-16
+
+# gen: generator arguments; lanes: (GJK, EPA) lanes per pair; cluster_min:
+# bodies with at least this many vertices are swept through their clusters
+# (None: no clusters); roofline: "compulsory" (every vertex once) or
+# "stream" (every body re-streamed once per oracle sweep); profile: the
+# rocprofv3 summary holding the measured HBM traffic of one step.
 WORKLOADS = {
-    "sphere64": dict(family=0, n_pairs=100_000, seed=1, nverts=64, box=1.29,
-                     desc="100000 pairs x 64-vertex sphere polytopes, radius U[0.5,1.5], "
-                          "centres U[-1.29,1.29]^3 (~60% penetrating), GJK then EPA"),
-    "ellipsoid_uneven": dict(family=2, n_pairs=1_000_000, seed=2, box=1.5, nmin=4, nmax=4096,
-                             desc="1000000 pairs, vertex counts log-uniform in [4,4096], "
-                                  "ellipsoids axes U[0.3,2] rotated, centres U[-1.5,1.5]^3, GJK then EPA"),
+    "sphere64": dict(
+        gen=dict(family=0, n_pairs=100_000, seed=1, nverts=64, box=1.29),
+        desc="100000 pairs x 64-vertex sphere polytopes, radius U[0.5,1.5], centres "
+             "U[-1.29,1.29]^3 (~60% penetrating), GJK then EPA",
+        lanes=(8, 32), cluster_min=None, roofline="compulsory",
+        profile="gjk_epa_sphere64_100k.md"),
+    "ellipsoid_uneven": dict(
+        gen=dict(family=2, n_pairs=1_000_000, seed=2, box=1.5, nmin=4, nmax=4096),
+        desc="1000000 pairs, vertex counts log-uniform in [4,4096], ellipsoids axes U[0.3,2] "
+             "rotated, centres U[-1.5,1.5]^3, GJK then EPA",
+        lanes=(64, 64), cluster_min=512, roofline="compulsory",
+        profile="gjk_epa_ellipsoid_uneven_1m.md"),
+    "noisy_sphere_large": dict(
+        gen=dict(family=3, n_pairs=10_000, seed=3, nverts=100_000, box=1.5, sigma=0.01),
+        desc="10000 pairs x 100000-vertex polytopes, unit-sphere points + N(0,0.01^2), centres "
+             "U[-1.5,1.5]^3 (~50% penetrating), GJK then EPA",
+        lanes=(64, 64), cluster_min=1024, roofline="stream",
+        profile="gjk_epa_noisy_sphere_100k.md"),
 }
-# lanes per pair (GJK, EPA) used when --lanes is not given
-DEFAULT_LANES = {"sphere64": "8,32", "ellipsoid_uneven": "64,64"}
-# rocprofv3 summary holding the measured HBM traffic of one step of the workload
-PROFILE = {"sphere64": "gjk_epa_sphere64_100k.md", "ellipsoid_uneven": "gjk_epa_ellipsoid_uneven_1m.md"}
 
 
 def measured_traffic(workload: str):
     """HBM bytes per step from the committed rocprofv3 summary (FETCH_SIZE and
     WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
-    path = os.path.join(ROOT, "profiles", PROFILE.get(workload, ""))
+    path = os.path.join(ROOT, "profiles", WORKLOADS[workload]["profile"])
     if not os.path.isfile(path):
         return None
     for line in open(path):
@@ -52,34 +75,44 @@ def measured_traffic(workload: str):
     return None
 
 
-def bytes_per_launch(batch, mode_epa=True) -> float:
-    """Algorithmic HBM bytes of one launch: every vertex read once (12 B FP32
-    SoA), the two body-table records per pair (off 8 + cnt 4 + r1 4 B each),
-    and the results written through the gkSimplex ABI (108 B simplex + 4 B
-    distance + 12 B normal)."""
+def compulsory_bytes(batch) -> float:
+    """Every vertex read once (12 B FP32 SoA), the two body-table records per
+    pair (off 8 + cnt 4 + r1 4 B each), and the results written through the
+    gkSimplex ABI (108 B simplex + 4 B distance + 12 B normal)."""
     verts = int(batch.cnt[batch.pa].astype(np.int64).sum() + batch.cnt[batch.pb].astype(np.int64).sum())
     n = batch.n_pairs
-    return 12.0 * verts + 32.0 * n + (124.0 if mode_epa else 112.0) * n
+    return 12.0 * verts + 32.0 * n + 124.0 * n
 
 
 def cpu_baseline(batch, gpu_out, budget_s: float = 20.0):
     """CPU baseline leg: the FP64 oracle (the reference CPU algorithm restated)
     on this box's cores, over a bounded prefix of the same workload.  The GPU
-    results for the same pairs are checked against it here as well."""
+    results for the same pairs are checked against it here as well.  Also
+    returns the oracle's stream bytes per pair (12 B x vertices x sweeps)."""
     import oracle
     from ogjk import gen
     cores = min(len(os.sched_getaffinity(0)), 16)
     n = batch.n_pairs
-    # size the sample from a quick probe so the leg stays within budget_s
-    probe = min(n, 2000)
-    sub = gen.Batch(batch.coords, batch.off, batch.cnt, batch.pa[:probe], batch.pb[:probe])
-    t = time.perf_counter()
-    oracle.run(3, sub.coords, sub.off, sub.cnt, sub.pa, sub.pb, bits=64, nthreads=1)
-    per_pair_1c = (time.perf_counter() - t) / probe
+
+    def prefix(m):
+        return gen.Batch(batch.coords, batch.off, batch.cnt, batch.pa[:m], batch.pb[:m])
+
+    # single-core probe, doubled until it takes 0.3 s, sizes the sample
+    probe, el1 = 8, 0.0
+    while True:
+        probe = min(n, probe * 2)
+        sub = prefix(probe)
+        t = time.perf_counter()
+        oracle.run(3, sub.coords, sub.off, sub.cnt, sub.pa, sub.pb, bits=64, nthreads=1)
+        el1 = time.perf_counter() - t
+        if el1 > 0.3 or probe == n:
+            break
+    per_pair_1c = el1 / probe
     m = int(min(n, max(probe, budget_s * 0.5 * cores / max(per_pair_1c, 1e-9))))
-    sub = gen.Batch(batch.coords, batch.off, batch.cnt, batch.pa[:m], batch.pb[:m])
+    sub = prefix(m)
     t = time.perf_counter()
-    ref = oracle.run(3, sub.coords, sub.off, sub.cnt, sub.pa, sub.pb, bits=64, nthreads=cores)
+    ref = oracle.run(3, sub.coords, sub.off, sub.cnt, sub.pa, sub.pb, bits=64, nthreads=cores,
+                     stats=True)
     el = time.perf_counter() - t
     # parity of the timed GPU results on the same pairs (FP64 records)
     rec = gpu_out["f64"][:m].cpu().numpy()
@@ -93,12 +126,16 @@ def cpu_baseline(batch, gpu_out, budget_s: float = 20.0):
     parity = {"pairs_checked": m, "max_rel_err_distance": float(np.nanmax(rel)),
               "max_abs_err_witness": float(np.nanmax(w)), "max_abs_err_normal": float(np.nanmax(nn)),
               "bit_exact_fraction": float(exact.mean()),
-              "overlap_fraction": float((dr <= 2.220446049250313e-16).mean())}
+              "overlap_fraction": float((dr <= EPS64).mean())}
+    st = ref.stats
+    na = batch.cnt[batch.pa[:m]].astype(np.float64)
+    nb = batch.cnt[batch.pb[:m]].astype(np.float64)
+    stream_per_pair = float((12.0 * (na * st["sweeps_a"] + nb * st["sweeps_b"])).mean())
     base = {"value": m / el, "unit": "pairs/s", "cores": cores, "kind": "port",
             "sample": f"first {m} pairs of the workload, FP64 oracle (restated reference CPU "
                       f"openGJK+EPA), {cores} threads, {el:.2f}s; single-core "
-                      f"{1.0 / per_pair_1c:.0f} pairs/s"}
-    return base, parity
+                      f"{1.0 / per_pair_1c:.1f} pairs/s"}
+    return base, parity, stream_per_pair
 
 
 def main() -> int:
@@ -114,6 +151,9 @@ def main() -> int:
                     help="also run a 256 MiB elementwise copy (reads and writes 256 MiB) so a "
                          "profiler's FETCH_SIZE/WRITE_SIZE can be scaled against known bytes")
     ap.add_argument("--lanes", default=None, help="GJK,EPA lanes per pair (default per workload)")
+    ap.add_argument("--cluster-min", type=int, default=None,
+                    help="cull bodies with at least this many vertices (0: no clusters; "
+                         "default per workload)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -128,8 +168,9 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         return 1
-    lanes = args.lanes or DEFAULT_LANES[args.workload]
-    lg, le = (int(x) for x in lanes.split(","))
+    spec = WORKLOADS[args.workload]
+    lg, le = (int(x) for x in args.lanes.split(",")) if args.lanes else spec["lanes"]
+    cmin = spec["cluster_min"] if args.cluster_min is None else (args.cluster_min or None)
     ogjk.set_config(lg, le, False)
     if args.calibrate:
         x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
@@ -137,8 +178,7 @@ def main() -> int:
             y = x * 1.0
         torch.cuda.synchronize()
         del x, y
-    wl = dict(WORKLOADS[args.workload])
-    desc = wl.pop("desc")
+    wl = dict(spec["gen"])
     if args.pairs:
         wl["n_pairs"] = args.pairs
     t0 = time.perf_counter()
@@ -147,6 +187,13 @@ def main() -> int:
           f"in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     n = batch.n_pairs
     scene = ogjk.DeviceScene(batch, f64=not args.no_plain)
+    build_s = None
+    if cmin:
+        t0 = time.perf_counter()
+        scene.build_clusters(cmin)
+        torch.cuda.synchronize()
+        build_s = time.perf_counter() - t0
+        print(f"# clusters built in {build_s:.2f}s", file=sys.stderr, flush=True)
     out = scene.outputs(n, f64=True)
     stream = torch.cuda.current_stream()
 
@@ -158,12 +205,20 @@ def main() -> int:
     torch.cuda.synchronize()
     timed_out = scene.outputs(n, f64=False)
 
-    def timed(plain: bool):
+    def timed(plain: bool, steps=None):
+        """Warm up, then time `steps` steps (None: as many as take about 4 s,
+        at least 2 and at most --steps)."""
         for _ in range(args.warmup):
             step(timed_out, plain)
         torch.cuda.synchronize()
+        if steps is None:
+            t = time.perf_counter()
+            step(timed_out, plain)
+            torch.cuda.synchronize()
+            one = time.perf_counter() - t
+            steps = int(max(2, min(args.steps, 4.0 / max(one, 1e-6))))
         evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-               for _ in range(args.steps)]
+               for _ in range(steps)]
         t = time.perf_counter()
         for a, b in evs:
             a.record(stream)
@@ -172,28 +227,32 @@ def main() -> int:
         torch.cuda.synchronize()
         wall = time.perf_counter() - t
         ms = np.array([a.elapsed_time(b) for a, b in evs])
-        return wall, ms
+        return wall, ms, steps
 
-    wall, ms = timed(False)
+    wall, ms, _ = timed(False, args.steps)
     print(f"# mixed: {wall / args.steps * 1e3:.3f} ms/step wall, kernel median {np.median(ms):.3f} ms",
           file=sys.stderr, flush=True)
     plain = pipe64 = None
     if not args.no_plain:
-        # the same split pipeline over FP64 vertex storage (isolates the FP32 sweep)
-        wall_q, ms_q = timed(True)
-        pipe64 = {"pairs_per_s": n * args.steps / wall_q, "ms_per_step": wall_q / args.steps * 1e3}
+        # comparison legs run fewer steps when a step is long (about 4 s each)
+        # the same split pipeline over FP64 vertex storage, no clusters
+        # (isolates the FP32 sweep and the culling)
+        wall_q, _, csteps = timed(True)
+        pipe64 = {"pairs_per_s": n * csteps / wall_q, "ms_per_step": wall_q / csteps * 1e3,
+                  "steps": csteps}
         print(f"# pipeline over fp64 storage: {pipe64['ms_per_step']:.3f} ms/step", file=sys.stderr,
               flush=True)
         # plain FP64 per-pair kernel: one group per pair runs GJK then EPA in one
-        # non-persistent launch over FP64 SoA vertices, no worklist; the faster
-        # of 32 and 64 lanes per pair is the denominator
+        # non-persistent launch over FP64 SoA vertices, no worklist, no
+        # clusters; the faster of 32 and 64 lanes per pair is the denominator
         plain = {"variants": {}}
         for el in (32, 64):
             ogjk.set_config(lg, el, True)
-            wall_p, ms_p = timed(True)
-            plain["variants"][f"{el}_lanes"] = {"pairs_per_s": n * args.steps / wall_p,
-                                                "ms_per_step": wall_p / args.steps * 1e3}
-            print(f"# plain fp64 per-pair kernel, {el} lanes: {wall_p / args.steps * 1e3:.3f} ms/step",
+            wall_p, _, psteps = timed(True)
+            plain["variants"][f"{el}_lanes"] = {"pairs_per_s": n * psteps / wall_p,
+                                                "ms_per_step": wall_p / psteps * 1e3,
+                                                "steps": psteps}
+            print(f"# plain fp64 per-pair kernel, {el} lanes: {wall_p / psteps * 1e3:.3f} ms/step",
                   file=sys.stderr, flush=True)
         # the plain kernel's results equal the mixed pipeline's, pair by pair
         chk = scene.outputs(n, f64=True)
@@ -206,16 +265,25 @@ def main() -> int:
         plain.update(best)
 
     value = n * args.steps / wall
-    algo = bytes_per_launch(batch)
     k_ms = float(np.mean(ms))
+    base = parity = stream_pp = None
+    if not args.no_cpu:
+        base, parity, stream_pp = cpu_baseline(batch, out)
+    if spec["roofline"] == "stream" and stream_pp is not None:
+        algo, algo_kind = stream_pp * n, ("stream: 12 B x body vertices x the FP64 oracle's sweeps "
+                                          "of that body, mean over the CPU sample x pairs")
+    else:
+        algo, algo_kind = compulsory_bytes(batch), "compulsory: every vertex once + records"
     achieved = algo / (k_ms * 1e-3) / 1e9
+    comp = compulsory_bytes(batch) / (k_ms * 1e-3) / 1e9
     line = {
         "metric": "GJK+EPA pairs/s vs plain FP64 GPU path at 1e-5 rel err; p50/p99 query latency",
         "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
-        "config": {"workload": desc, "pairs": n, "vertices": int(batch.coords.shape[0]),
+        "config": {"workload": spec["desc"], "pairs": n, "vertices": int(batch.coords.shape[0]),
                    "lanes_per_pair": {"gjk": lg, "epa": le},
+                   "clusters": {"min_verts": cmin, "build_s": build_s} if cmin else None,
                    "layout": "SoA fp32 scene; GJK kernel -> device worklist -> persistent EPA kernel"},
         "step_ms_p50": float(np.percentile(ms, 50)), "step_ms_p99": float(np.percentile(ms, 99)),
         "plain_f64": plain,
@@ -224,10 +292,13 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
                      "kernel": "k_gjk + k_epa_list (one step)",
-                     "kernel_ms_mean": k_ms, "algorithmic_bytes_per_launch": algo},
+                     "kernel_ms_mean": k_ms, "algorithmic_bytes_per_launch": algo,
+                     "algorithmic_bytes": algo_kind,
+                     # every vertex read once: a floor on HBM traffic, so this
+                     # fraction stays <= 1 where culling beats the stream figure
+                     "achieved_compulsory": comp, "frac_compulsory": comp / HBM_PEAK_GBS},
     }
-    if not args.no_cpu:
-        base, parity = cpu_baseline(batch, out)
+    if base is not None:
         line["cpu_baseline"] = base
         line["parity"] = parity
     print(json.dumps(line), flush=True)
