@@ -53,7 +53,7 @@ WORKLOADS = {
         desc="1000000 pairs, vertex counts log-uniform in [4,4096], ellipsoids axes U[0.3,2] "
              "rotated, centres U[-1.5,1.5]^3, GJK then EPA",
         lanes=(64, 64), cluster_min=512, roofline="compulsory",
-        profile="gjk_epa_ellipsoid_uneven_1m.md"),
+        profile="gjk_epa_ellipsoid_uneven_1m.md", tuning="ellipsoid_uneven_1m.json"),
     "noisy_sphere_large": dict(
         gen=dict(family=3, n_pairs=10_000, seed=3, nverts=100_000, box=1.5, sigma=0.01),
         desc="10000 pairs x 100000-vertex polytopes, unit-sphere points + N(0,0.01^2), centres "
@@ -168,9 +168,17 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         return 1
+    from ogjk import autotune
     spec = WORKLOADS[args.workload]
     lg, le = (int(x) for x in args.lanes.split(",")) if args.lanes else spec["lanes"]
     cmin = spec["cluster_min"] if args.cluster_min is None else (args.cluster_min or None)
+    # a size-bucket table written by the autotuner, unless lanes are forced
+    table = None
+    tpath = os.path.join(ROOT, "opengjk-gpu_amd", "tuning", spec.get("tuning") or "-")
+    if not args.lanes and os.path.isfile(tpath):
+        table = autotune.load(tpath)
+        if args.cluster_min is None:
+            cmin = table["cluster_min"]
     ogjk.set_config(lg, le, False)
     if args.calibrate:
         x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
@@ -194,6 +202,8 @@ def main() -> int:
         torch.cuda.synchronize()
         build_s = time.perf_counter() - t0
         print(f"# clusters built in {build_s:.2f}s", file=sys.stderr, flush=True)
+    if table is not None:
+        autotune.apply(table)
     out = scene.outputs(n, f64=True)
     stream = torch.cuda.current_stream()
 
@@ -261,6 +271,8 @@ def main() -> int:
         plain["identical_to_mixed_fraction"] = float((chk["f64"] == out["f64"]).all(dim=1).float().mean())
         del chk
         ogjk.set_config(lg, le, False)
+        if table is not None:
+            autotune.apply(table)
         best = max(plain["variants"].values(), key=lambda v: v["pairs_per_s"])
         plain.update(best)
 
@@ -282,7 +294,10 @@ def main() -> int:
         "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": spec["desc"], "pairs": n, "vertices": int(batch.coords.shape[0]),
-                   "lanes_per_pair": {"gjk": lg, "epa": le},
+                   "lanes_per_pair": ({"buckets_max_verts": table["max_verts"],
+                                       "gjk": table["gjk_lanes"], "epa": table["epa_lanes"],
+                                       "table": "opengjk-gpu_amd/tuning/" + spec["tuning"]}
+                                      if table else {"gjk": lg, "epa": le}),
                    "clusters": {"min_verts": cmin, "build_s": build_s} if cmin else None,
                    "layout": "SoA fp32 scene; GJK kernel -> device worklist -> persistent EPA kernel"},
         "step_ms_p50": float(np.percentile(ms, 50)), "step_ms_p99": float(np.percentile(ms, 99)),

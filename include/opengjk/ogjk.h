@@ -138,6 +138,21 @@ ogjk_status ogjk_collide_scene_ws(int mode, const ogjk_scene* scene, const int32
                                   float* d_normals, double* d_f64, void* d_workspace,
                                   size_t workspace_bytes, void* stream);
 
+/* A GJK / GJK+EPA launch over a scene captured as one HIP graph: the whole
+ * sequence (counter reset, size buckets, GJK, worklist, persistent EPA) is
+ * replayed by a single ogjk_graph_launch, with no host work per batch.
+ * Pointers, the pair count and the launch configuration are fixed at
+ * capture; the data behind the pointers (coordinates, pair lists) may change
+ * between launches.  The graph owns its workspace, so launches of different
+ * graphs may overlap; launches of one graph must not.  No counterpart in
+ * the reference (its launches are synchronous, GJK/gpu/openGJK.cu:2968-3003). */
+typedef struct ogjk_graph_s* ogjk_graph;
+ogjk_status ogjk_graph_capture_scene(int mode, const ogjk_scene* scene, const int32_t* d_pairs,
+                                     int32_t n_pairs, gkSimplex* d_simplices, float* d_distances,
+                                     float* d_normals, double* d_f64, ogjk_graph* graph);
+ogjk_status ogjk_graph_launch(ogjk_graph graph, void* stream);
+ogjk_status ogjk_graph_destroy(ogjk_graph graph);
+
 /* Slots a cluster build needs for a scene of n_verts vertices in n_bodies
  * bodies (padding to 32 per body included). */
 int64_t ogjk_cluster_slots(int64_t n_verts, int32_t n_bodies);

@@ -5,6 +5,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -249,6 +250,31 @@ ogjk_status ogjk_pack_scene_f64(const float* d_coords_aos, int64_t n_verts, doub
   return OGJK_OK;
 }
 
+// Validates a float scene and its optional clusters into launch arguments.
+static ogjk_status scene_args(const ogjk_scene* scene, const int32_t* d_pairs, int32_t n_pairs,
+                              ogjk::SceneArgs* sa) {
+  if (!scene || !scene->x || !scene->y || !scene->z || !scene->body_off || !scene->body_cnt ||
+      !scene->body_r1)
+    return fail(OGJK_ERR_INVALID, "incomplete scene");
+  if (!d_pairs && (int64_t)scene->n_bodies < 2 * (int64_t)n_pairs)
+    return fail(OGJK_ERR_INVALID, "implicit pairs need %lld bodies, scene has %d",
+                2 * (long long)n_pairs, scene->n_bodies);
+  *sa = ogjk::SceneArgs{scene->x, scene->y, scene->z, scene->body_off, scene->body_cnt,
+                        scene->body_r1, scene->n_bodies, d_pairs};
+  if (const ogjk_clusters* cl = scene->clusters) {
+    if (!cl->x || !cl->y || !cl->z || !cl->orig || !cl->body_slot || !cl->bounds)
+      return fail(OGJK_ERR_INVALID, "incomplete clusters");
+    sa->cx = cl->x;
+    sa->cy = cl->y;
+    sa->cz = cl->z;
+    sa->corig = cl->orig;
+    sa->cslot = cl->body_slot;
+    sa->cbounds = cl->bounds;
+    sa->cluster_min = cl->min_verts > 1 ? cl->min_verts : 1;
+  }
+  return OGJK_OK;
+}
+
 static ogjk_status check_out(int mode, int32_t n, gkSimplex* s, float* d, float* nrm) {
   if (!valid_mode(mode)) return fail(OGJK_ERR_INVALID, "mode %d is not 1, 2 or 3", mode);
   if (n < 0) return fail(OGJK_ERR_INVALID, "n = %d < 0", n);
@@ -264,30 +290,67 @@ ogjk_status ogjk_collide_scene_ws(int mode, const ogjk_scene* scene, const int32
   const ogjk_status st = check_out(mode, n_pairs, d_simplices, d_distances, d_normals);
   if (st != OGJK_OK) return st;
   if (n_pairs == 0) return OGJK_OK;
-  if (!scene || !scene->x || !scene->y || !scene->z || !scene->body_off || !scene->body_cnt ||
-      !scene->body_r1)
-    return fail(OGJK_ERR_INVALID, "incomplete scene");
-  if (!d_pairs && (int64_t)scene->n_bodies < 2 * (int64_t)n_pairs)
-    return fail(OGJK_ERR_INVALID, "implicit pairs need %lld bodies, scene has %d",
-                2 * (long long)n_pairs, scene->n_bodies);
   if (mode != OGJK_MODE_EPA && (!d_workspace || workspace_bytes < ogjk::workspace_bytes(n_pairs)))
     return fail(OGJK_ERR_INVALID, "workspace of %zu bytes needed", ogjk::workspace_bytes(n_pairs));
-  ogjk::SceneArgs sa{scene->x, scene->y, scene->z, scene->body_off, scene->body_cnt,
-                     scene->body_r1, scene->n_bodies, d_pairs};
-  if (const ogjk_clusters* cl = scene->clusters) {
-    if (!cl->x || !cl->y || !cl->z || !cl->orig || !cl->body_slot || !cl->bounds)
-      return fail(OGJK_ERR_INVALID, "incomplete clusters");
-    sa.cx = cl->x;
-    sa.cy = cl->y;
-    sa.cz = cl->z;
-    sa.corig = cl->orig;
-    sa.cslot = cl->body_slot;
-    sa.cbounds = cl->bounds;
-    sa.cluster_min = cl->min_verts > 1 ? cl->min_verts : 1;
-  }
+  ogjk::SceneArgs sa;
+  const ogjk_status s2 = scene_args(scene, d_pairs, n_pairs, &sa);
+  if (s2 != OGJK_OK) return s2;
   const ogjk::OutArgs oa{d_simplices, d_distances, d_normals, d_f64};
   OGJK_HIP(ogjk::launch_scene(mode, sa, oa, n_pairs, d_workspace, (hipStream_t)stream),
            "kernel launch");
+  return OGJK_OK;
+}
+
+// ------------------------------------------------------------ HIP graphs
+
+struct ogjk_graph_s {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void* workspace = nullptr;
+  ~ogjk_graph_s() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (workspace) (void)hipFree(workspace);
+  }
+};
+
+ogjk_status ogjk_graph_capture_scene(int mode, const ogjk_scene* scene, const int32_t* d_pairs,
+                                     int32_t n_pairs, gkSimplex* d_simplices, float* d_distances,
+                                     float* d_normals, double* d_f64, ogjk_graph* out) {
+  if (!out) return fail(OGJK_ERR_INVALID, "NULL graph handle");
+  *out = nullptr;
+  const ogjk_status st = check_out(mode, n_pairs, d_simplices, d_distances, d_normals);
+  if (st != OGJK_OK) return st;
+  ogjk::SceneArgs sa;
+  const ogjk_status s2 = scene_args(scene, d_pairs, n_pairs, &sa);
+  if (s2 != OGJK_OK) return s2;
+  auto g = std::make_unique<ogjk_graph_s>();
+  const size_t ws_bytes = ogjk::workspace_bytes(n_pairs);
+  OGJK_HIP(hipMalloc(&g->workspace, ws_bytes), "hipMalloc(graph workspace)");
+  hipStream_t cs = nullptr;
+  OGJK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreate");
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    const ogjk::OutArgs oa{d_simplices, d_distances, d_normals, d_f64};
+    const hipError_t le = ogjk::launch_scene(mode, sa, oa, n_pairs, g->workspace, cs);
+    e = hipStreamEndCapture(cs, &g->graph);
+    if (le != hipSuccess) e = le;
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  (void)hipStreamDestroy(cs);
+  OGJK_HIP(e, "graph capture");
+  *out = g.release();
+  return OGJK_OK;
+}
+
+ogjk_status ogjk_graph_launch(ogjk_graph g, void* stream) {
+  if (!g || !g->exec) return fail(OGJK_ERR_INVALID, "NULL graph");
+  OGJK_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream), "hipGraphLaunch");
+  return OGJK_OK;
+}
+
+ogjk_status ogjk_graph_destroy(ogjk_graph g) {
+  delete g;
   return OGJK_OK;
 }
 

@@ -275,13 +275,20 @@ __global__ void __launch_bounds__(256) k_pairs(Src src, OutArgs out, int n) {
 // ------------------------------------------------- size-bucketed pipeline
 //
 // Workspace (see workspace_bytes): 8 KiB of counters, then the pair list
-// (int32, bucket b at [start_b, start_b + count_b)), then the EPA jobs (one
-// 192-B record per penetrating pair, grouped by EPA lane class).
-//   counters[0..3]   pairs per bucket        counters[4..7]   scatter cursors
-//   counters[12..14] jobs per EPA class
-//   counters[64 + (q*8 + s)*32]  claim cursor of slice s of queue q
-//                    (q 0..3: GJK buckets, q 4..6: EPA classes)
-enum { kCntBucket = 0, kCntScatter = 4, kCntJobs = 12 };
+// (int32, bucket b at [start_b, start_b + count_b)), the per-block bucket
+// counts of the partition, then the EPA jobs (one 192-B record per
+// penetrating pair, grouped by EPA lane class).  Counters, in 4-byte words:
+//   [0..3]   pairs per bucket      [4..7]  first list slot of each bucket
+//   [8..10]  first job slot of each EPA class
+//            (written once per launch by k_scan_blocks, read-only after)
+//   [32 * (1 + cls)]  jobs emitted for EPA class cls (one 128-B line each)
+//   [kClaimBase + (q*8 + s)*32]  claim cursor of slice s of queue q
+//            (q 0..3: GJK buckets, q 4..6: EPA classes)
+// Words that take atomics never share a 128-B line with words other waves
+// read: a device-scope atomic and a load on one line serialise across XCDs.
+enum { kCntBucket = 0, kBucketStart = 4, kClassStart = 8, kJobLine = 32 };
+
+OGJK_DEV unsigned* job_counter(unsigned* c, int cls) { return c + kJobLine * (1 + cls); }
 
 OGJK_DEV int bucket_of(const BucketPlan& p, int size) {
   int b = 0;
@@ -289,21 +296,6 @@ OGJK_DEV int bucket_of(const BucketPlan& p, int size) {
   for (int k = 0; k < kMaxBuckets - 1; k++)
     if (k < p.nb - 1 && size > p.max_verts[k]) b = k + 1;
   return b;
-}
-
-OGJK_DEV unsigned bucket_start(const unsigned* c, int b) {
-  unsigned s = 0;
-  for (int k = 0; k < b; k++) s += __atomic_load_n(&c[kCntBucket + k], __ATOMIC_RELAXED);
-  return s;
-}
-
-// First job slot of EPA class `cls`: classes are laid out in order, each sized
-// by the pairs of the buckets that feed it.
-OGJK_DEV unsigned class_start(const unsigned* c, const BucketPlan& p, int cls) {
-  unsigned s = 0;
-  for (int k = 0; k < p.nb; k++)
-    if (p.epa_class[k] < cls) s += __atomic_load_n(&c[kCntBucket + k], __ATOMIC_RELAXED);
-  return s;
 }
 
 // Device-scope atomics on one address serialise across the XCDs, so every
@@ -326,9 +318,9 @@ OGJK_DEV unsigned wave_fetch_add(unsigned* ctr, bool pred) {
 // starts on the slice of its block and moves on when a slice runs dry.
 constexpr int kSlices = 8;
 constexpr int kLineU = 32;  // uints per 128-B line
-constexpr int kClaimBase = 64;
+constexpr int kClaimBase = 256;
 constexpr int kQueueEpa = 4;
-constexpr size_t kCounterBytes = 8192;  // >= (kClaimBase + 7*kSlices*kLineU) * 4
+constexpr size_t kCounterBytes = 8192;  // = (kClaimBase + 7*kSlices*kLineU) * 4
 
 // Claim the next item of [0, count) for every lane with `want` (group
 // leaders).  Returns the item, or 0xffffffff when all slices are exhausted.
@@ -359,21 +351,94 @@ OGJK_DEV unsigned claim(unsigned* c, int queue, unsigned count, int& slice, int&
   return result;
 }
 
+// Stable partition of the pairs by bucket, in three kernels: per-block
+// counts, one exclusive scan over blocks, scatter.  The list keeps the input
+// order inside each bucket, so concurrent waves work on neighbouring pairs
+// (and neighbouring pages) as in the unbucketed launch.
 template <class Src>
-__global__ void __launch_bounds__(256) k_classify(Src src, int n, BucketPlan plan, unsigned* c) {
+__global__ void __launch_bounds__(256) k_classify(Src src, int n, BucketPlan plan, unsigned* blk,
+                                                  int nblk) {
+  __shared__ unsigned wc[4][kMaxBuckets];
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int b = i < n ? bucket_of(plan, src.pair_size(i)) : -1;
-  for (int k = 0; k < plan.nb; k++) (void)wave_fetch_add(&c[kCntBucket + k], b == k);
+  const int w = threadIdx.x >> 6;
+  for (int k = 0; k < plan.nb; k++) {
+    const unsigned cnt = (unsigned)__popcll(__ballot(b == k));
+    if ((threadIdx.x & 63) == 0) wc[w][k] = cnt;
+  }
+  __syncthreads();
+  if (threadIdx.x < plan.nb) {
+    const int k = threadIdx.x;
+    blk[(size_t)k * nblk + blockIdx.x] = wc[0][k] + wc[1][k] + wc[2][k] + wc[3][k];
+  }
+}
+
+// One block: exclusive scan of each bucket's block counts; bucket totals go
+// to the counters.
+__global__ void __launch_bounds__(1024) k_scan_blocks(unsigned* blk, int nblk, BucketPlan plan,
+                                                      unsigned* c) {
+  const int nb = plan.nb;
+  __shared__ unsigned wsum[16];
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int k = 0; k < nb; k++) {
+    unsigned* a = blk + (size_t)k * nblk;
+    unsigned carry = 0;
+    for (int base = 0; base < nblk; base += 1024) {
+      const int i = base + threadIdx.x;
+      const unsigned v = i < nblk ? a[i] : 0u;
+      unsigned x = v;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const unsigned y = __shfl_up(x, d, 64);
+        if (l >= d) x += y;
+      }
+      if (l == 63) wsum[w] = x;
+      __syncthreads();
+      unsigned woff = 0, total = 0;
+      for (int q = 0; q < 16; q++) {
+        woff += q < w ? wsum[q] : 0u;
+        total += wsum[q];
+      }
+      if (i < nblk) a[i] = carry + woff + x - v;
+      carry += total;
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) c[kCntBucket + k] = carry;
+  }
+  if (threadIdx.x == 0) {
+    unsigned start = 0;
+    for (int k = 0; k < nb; k++) {
+      c[kBucketStart + k] = start;
+      start += c[kCntBucket + k];
+    }
+    for (int cls = 0; cls < 3; cls++) {
+      unsigned cs = 0;
+      for (int k = 0; k < nb; k++)
+        if (plan.epa_class[k] < cls) cs += c[kCntBucket + k];
+      c[kClassStart + cls] = cs;
+    }
+  }
 }
 
 template <class Src>
-__global__ void __launch_bounds__(256) k_scatter(Src src, int n, BucketPlan plan, unsigned* c,
+__global__ void __launch_bounds__(256) k_scatter(Src src, int n, BucketPlan plan,
+                                                 const unsigned* c, const unsigned* blk, int nblk,
                                                  int* list) {
+  __shared__ unsigned wc[4][kMaxBuckets];
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int b = i < n ? bucket_of(plan, src.pair_size(i)) : -1;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  unsigned rank = 0;
   for (int k = 0; k < plan.nb; k++) {
-    const unsigned slot = wave_fetch_add(&c[kCntScatter + k], b == k);
-    if (b == k) list[bucket_start(c, k) + slot] = i;
+    const uint64_t m = __ballot(b == k);
+    if (l == 0) wc[w][k] = (unsigned)__popcll(m);
+    if (b == k) rank = (unsigned)__popcll(m & ((1ull << l) - 1ull));
+  }
+  __syncthreads();
+  if (b >= 0) {
+    unsigned off = c[kBucketStart + b] + blk[(size_t)b * nblk + blockIdx.x] + rank;
+    for (int q = 0; q < w; q++) off += wc[q][b];
+    list[off] = i;
   }
 }
 
@@ -391,11 +456,14 @@ __global__ void __launch_bounds__(64, OGJK_GJK_WAVES)
           EpaJob* jobs) {
   constexpr int GPB = 64 / LPP;
   const int lane = threadIdx.x & (LPP - 1);
-  const unsigned count = __builtin_amdgcn_readfirstlane(
-      plan.nb > 1 ? __atomic_load_n(&c[kCntBucket + bucket], __ATOMIC_RELAXED) : (unsigned)n);
+  // the partition's results are read-only here (plain, cached loads)
+  const bool multi = plan.nb > 1;
+  const unsigned count =
+      __builtin_amdgcn_readfirstlane(multi ? c[kCntBucket + bucket] : (unsigned)n);
   const unsigned j = blockIdx.x * GPB + threadIdx.x / LPP;
   if (j >= count) return;
-  const int pair = plan.nb > 1 ? list[bucket_start(c, bucket) + j] : (int)j;
+  const int pair = multi ? list[__builtin_amdgcn_readfirstlane(c[kBucketStart + bucket]) + j]
+                         : (int)j;
   typename Src::BodyT A, B;
   const bool ok = src.template load<LPP>(pair, lane, A, B) && A.n > 0 && B.n > 0;
   if (!ok) {
@@ -407,10 +475,10 @@ __global__ void __launch_bounds__(64, OGJK_GJK_WAVES)
   const double dist = gjk<LPP>(A, B, s, w, lane);
   const bool pen = EMIT && !(dist > kEps64);
   const int cls = plan.epa_class[bucket];
-  const unsigned k = wave_fetch_add(&c[kCntJobs + cls], lane == 0 && pen);
+  const unsigned k = wave_fetch_add(job_counter(c, cls), lane == 0 && pen);
   if (lane != 0) return;
   if (pen) {
-    job_store(jobs + class_start(c, plan, cls) + k, pair, s, w, dist);
+    job_store(jobs + (multi ? c[kClassStart + cls] : 0u) + k, pair, s, w, dist);
   } else {
     const D3 normal = EMIT ? contact_normal(w.w0, w.w1) : D3{0.0, 0.0, 0.0};
     store(out, pair, s, w, dist, normal, EMIT);
@@ -427,9 +495,10 @@ __global__ void __launch_bounds__(256, OGJK_EPA_WAVES)
   const int lane = threadIdx.x & (LPP - 1);
   const int g = threadIdx.x / LPP;
   const unsigned njobs =
-      __builtin_amdgcn_readfirstlane(__atomic_load_n(&c[kCntJobs + cls], __ATOMIC_RELAXED));
+      __builtin_amdgcn_readfirstlane(__atomic_load_n(job_counter(c, cls), __ATOMIC_RELAXED));
   if (blockIdx.x * GPB >= njobs) return;
-  const EpaJob* mine = jobs + __builtin_amdgcn_readfirstlane(class_start(c, plan, cls));
+  const EpaJob* mine =
+      jobs + __builtin_amdgcn_readfirstlane(plan.nb > 1 ? c[kClassStart + cls] : 0u);
   int slice = blockIdx.x & (kSlices - 1), tried = 0;
   for (;;) {
     unsigned j = claim(c, kQueueEpa + cls, njobs, slice, tried, lane == 0);
@@ -521,9 +590,17 @@ static BucketPlan plan_of(const LaunchConfig& c) {
 
 static size_t up256(size_t b) { return (b + 255) & ~size_t(255); }
 
+// Workspace pieces: counters | pair list | per-block bucket counts | EPA jobs.
+static size_t ws_list_off() { return kCounterBytes; }
+static int ws_blocks(int n) { return (n + 255) / 256; }
+static size_t ws_blk_off(int n) { return kCounterBytes + up256(4 * (size_t)n); }
+static size_t ws_jobs_off(int n) {
+  return ws_blk_off(n) + up256(4 * (size_t)kMaxBuckets * (size_t)ws_blocks(n));
+}
+
 size_t workspace_bytes(int n) {
-  const size_t m = (size_t)(n > 0 ? n : 0);
-  return kCounterBytes + up256(4 * m) + m * sizeof(EpaJob);
+  const int m = n > 0 ? n : 0;
+  return ws_jobs_off(m) + (size_t)m * sizeof(EpaJob);
 }
 
 template <int LPP, int MODE, class Src>
@@ -544,8 +621,8 @@ template <int LPP, bool EMIT, class Src>
 static hipError_t launch_gjk(const Src& src, const OutArgs& o, int n, const BucketPlan& p, int b,
                              void* ws, hipStream_t st) {
   unsigned* c = (unsigned*)ws;
-  const int* list = (const int*)((char*)ws + kCounterBytes);
-  EpaJob* jobs = (EpaJob*)((char*)ws + kCounterBytes + up256(4 * (size_t)n));
+  const int* list = (const int*)((char*)ws + ws_list_off());
+  EpaJob* jobs = (EpaJob*)((char*)ws + ws_jobs_off(n));
   constexpr int GPB = 64 / LPP;
   hipLaunchKernelGGL((k_gjk<LPP, EMIT, Src>), dim3((unsigned)((n + GPB - 1) / GPB)), dim3(64), 0,
                      st, src, o, n, p, b, c, list, jobs);
@@ -556,7 +633,7 @@ template <int LPP, class Src>
 static hipError_t launch_epa_list(const Src& src, const OutArgs& o, int n, const BucketPlan& p,
                                   int cls, void* ws, hipStream_t st) {
   unsigned* c = (unsigned*)ws;
-  const EpaJob* jobs = (const EpaJob*)((char*)ws + kCounterBytes + up256(4 * (size_t)n));
+  const EpaJob* jobs = (const EpaJob*)((char*)ws + ws_jobs_off(n));
   hipLaunchKernelGGL((k_epa_list<LPP, Src>), dim3(persistent_blocks(n, 256 / LPP)), dim3(256), 0,
                      st, src, o, p, cls, c, jobs);
   return hipGetLastError();
@@ -594,10 +671,12 @@ static hipError_t launch_bucketed(int mode, const Src& src, const OutArgs& o, in
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (p.nb > 1) {
-    const unsigned blocks = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL((k_classify<Src>), dim3(blocks), dim3(256), 0, st, src, n, p, c);
-    hipLaunchKernelGGL((k_scatter<Src>), dim3(blocks), dim3(256), 0, st, src, n, p, c,
-                       (int*)((char*)ws + kCounterBytes));
+    const int blocks = ws_blocks(n);
+    unsigned* blk = (unsigned*)((char*)ws + ws_blk_off(n));
+    hipLaunchKernelGGL((k_classify<Src>), dim3(blocks), dim3(256), 0, st, src, n, p, blk, blocks);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, blk, blocks, p, c);
+    hipLaunchKernelGGL((k_scatter<Src>), dim3(blocks), dim3(256), 0, st, src, n, p, c, blk, blocks,
+                       (int*)((char*)ws + ws_list_off()));
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   for (int b = 0; b < p.nb; b++) {

@@ -39,6 +39,7 @@ C_ABI_SYMBOLS = [
     "ogjk_collide_scene", "ogjk_pack_scene_f64", "ogjk_collide_scene_f64",
     "ogjk_collide_polytopes", "ogjk_collide_polytopes_indexed", "ogjk_collide_host",
     "ogjk_collide_host_indexed", "ogjk_cluster_slots", "ogjk_build_clusters",
+    "ogjk_graph_capture_scene", "ogjk_graph_launch", "ogjk_graph_destroy",
 ]
 
 
@@ -87,6 +88,10 @@ def lib():
             "ogjk_pack_scene": (C.c_int, [vp, vp, vp, i32, i64, vp, vp, vp, vp, vp]),
             "ogjk_pack_scene_f64": (C.c_int, [vp, i64, vp, vp, vp, vp]),
             "ogjk_cluster_slots": (i64, [i64, i32]),
+            "ogjk_graph_capture_scene": (C.c_int, [C.c_int, C.POINTER(_Scene), vp, i32, vp, vp, vp,
+                                                   vp, C.POINTER(vp)]),
+            "ogjk_graph_launch": (C.c_int, [vp, vp]),
+            "ogjk_graph_destroy": (C.c_int, [vp]),
             "ogjk_build_clusters": (C.c_int, [C.POINTER(_Scene), C.POINTER(_Clusters), vp]),
             "ogjk_collide_scene": (C.c_int, [C.c_int, C.POINTER(_Scene), vp, i32, vp, vp, vp, vp, vp]),
             "ogjk_collide_scene_f64": (C.c_int, [C.c_int, C.POINTER(_Scene64), vp, i32, vp, vp, vp, vp, vp]),
@@ -236,6 +241,11 @@ class DeviceScene:
             raise OgjkError("no clusters built")
         self.clusters.min_verts = int(min_verts)
 
+    def capture(self, mode: int, n_pairs: int, pairs=None, out=None, f64_record: bool = False):
+        """Capture one launch over this scene as a HIP graph (see Graph)."""
+        out = out or self.outputs(n_pairs, f64_record)
+        return Graph(self, mode, n_pairs, pairs, out), out
+
     def drop_clusters(self) -> None:
         """Sweep every body without culling."""
         self.scene.clusters = None
@@ -274,6 +284,34 @@ class DeviceScene:
                                             out["simplices"].data_ptr(), out["distances"].data_ptr(),
                                             out["normals"].data_ptr(), f, st))
         return out
+
+
+class Graph:
+    """A captured scene launch (ogjk_graph_capture_scene); replay with launch()."""
+
+    def __init__(self, scene, mode: int, n_pairs: int, pairs, out):
+        self._scene, self._out, self._pairs = scene, out, pairs  # keep buffers alive
+        h = C.c_void_p()
+        f = out["f64"].data_ptr() if out.get("f64") is not None else None
+        _check(lib().ogjk_graph_capture_scene(
+            mode, C.byref(scene.scene), None if pairs is None else pairs.data_ptr(), n_pairs,
+            out["simplices"].data_ptr(), out["distances"].data_ptr(), out["normals"].data_ptr(), f,
+            C.byref(h)))
+        self._h = h
+
+    def launch(self, stream=None) -> None:
+        _check(lib().ogjk_graph_launch(self._h, self._scene._stream(stream)))
+
+    def close(self) -> None:
+        if self._h is not None and self._h.value:
+            _check(lib().ogjk_graph_destroy(self._h))
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def simplices_from_device(t) -> np.ndarray:

@@ -112,6 +112,22 @@ def test_cluster_layout(torch_cuda):
             assert (np.abs(q - ctr) <= h).all()
 
 
+# Size-bucketed launch tables (device partition, per-bucket lanes, EPA lane
+# classes), with and without clusters, against the oracle.
+@pytest.mark.parametrize("table", [([64], [8, 64], [32, 64]), ([1], [32, 64], [32, 64]),
+                                   ([32, 256, 1024], [8, 16, 32, 64], [16, 32, 64, 64])])
+@pytest.mark.parametrize("cmin", [None, 256])
+def test_bucketed_parity(torch_cuda, table, cmin):
+    b = gen.make(gen.ELLIPSOID, 3000, seed=31, box=1.5, nmin=4, nmax=4096)
+    ref = oracle_run(oracle.GJK_EPA, b)
+    ogjk.set_buckets(*table)
+    try:
+        rec, simp, _, _ = run_scene(torch_cuda, b, oracle.GJK_EPA, cluster_min=cmin)
+    finally:
+        ogjk.set_config(8, 32, False)
+    check(compare_f64(rec, simp, ref, oracle.GJK_EPA), f"buckets {table} clusters {cmin}")
+
+
 def test_plain_f64_equals_mixed(torch_cuda):
     b = gen.make(gen.SPHERE, 2048, seed=11, nverts=64, box=1.3)
     rec_m, simp_m, _, _ = run_scene(torch_cuda, b, oracle.GJK_EPA)
