@@ -138,12 +138,162 @@ def cpu_baseline(batch, gpu_out, budget_s: float = 20.0):
     return base, parity, stream_per_pair
 
 
+def serving(args) -> int:
+    """Serving leg: a resident scene of 20,000 bodies (8-512 vertices,
+    log-uniform, ellipsoids in their local frames); requests arrive by a
+    Poisson process at --qps, each with K ~ U{1..10000} pairs (i != j
+    uniform) and a fresh pose per pair side (uniform unit quaternion,
+    translation U[-10,10]^3); GJK+EPA through the serving loop.  Latency is
+    submit to results in host memory."""
+    import threading
+    import torch
+    import ogjk
+    from ogjk import gen
+    ogjk.set_config(16, 32, False)
+    b = gen.make(gen.ELLIPSOID, 10_000, seed=4, box=0.0, nmin=8, nmax=512,
+                 nthreads=min(len(os.sched_getaffinity(0)), 16))
+    scene = ogjk.DeviceScene(b)
+    nb = len(b.cnt)
+    n_req = args.requests
+    reqs = gen.posed_requests(nb, n_req, seed=4)
+    total_pairs = int(sum(len(p) for p, _ in reqs))
+    print(f"# scene {nb} bodies, {b.coords.shape[0]} vertices; {n_req} requests, {total_pairs} pairs",
+          file=sys.stderr, flush=True)
+
+    # kernel roofline: one device launch of 10,000 posed pairs, HIP events
+    kp, kq = reqs[0]
+    while len(kp) < 10_000:
+        extra = gen.posed_requests(nb, 1, seed=len(kp) + 99)[0]
+        kp, kq = np.concatenate([kp, extra[0]]), np.concatenate([kq, extra[1]])
+    kp, kq = torch.from_numpy(kp[:10_000]).cuda(), torch.from_numpy(kq[:10_000]).cuda()
+    kout = scene.collide_posed(3, kp, kq)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(20):
+        scene.collide_posed(3, kp, kq, out=kout)
+    ev[1].record()
+    torch.cuda.synchronize()
+    k_ms = ev[0].elapsed_time(ev[1]) / 20
+    cnt = b.cnt.astype(np.int64)
+    kpn = kp.cpu().numpy()
+    k_bytes = float(12 * (cnt[kpn[:, 0]] + cnt[kpn[:, 1]]).sum() + 10_000 * (8 + 56 + 32 + 124))
+
+    def run_at(qps, rng_seed):
+        srv = ogjk.Server(scene, max_batch_pairs=65536, slots=4)
+        rng = np.random.default_rng(rng_seed)
+        gaps = rng.exponential(1.0 / qps, size=n_req)
+        arrive = np.cumsum(gaps)
+        tickets = [None] * n_req
+        done_ev = threading.Event()
+
+        def submitter():
+            t0 = time.perf_counter()
+            for k in range(n_req):
+                while time.perf_counter() - t0 < arrive[k]:
+                    pass
+                tickets[k] = srv.submit(*reqs[k])
+            done_ev.set()
+
+        th = threading.Thread(target=submitter)
+        t_start = time.perf_counter()
+        th.start()
+        lats = np.zeros(n_req)
+        for k in range(n_req):
+            while tickets[k] is None:
+                time.sleep(0)
+            _, lat, _ = srv.wait(tickets[k])
+            lats[k] = lat
+        wall = time.perf_counter() - t_start
+        th.join()
+        info = srv.info()
+        srv.close()
+        return {"qps": qps, "p50_ms": float(np.percentile(lats, 50) * 1e3),
+                "p99_ms": float(np.percentile(lats, 99) * 1e3),
+                "max_ms": float(lats.max() * 1e3), "pairs_per_s": total_pairs / wall,
+                "launches": info["launches"], "slots": info["slots"], "wall_s": wall}
+
+    sweep = []
+    for qps in sorted(set([args.qps] + [250, 500, 2000, 4000, 6000])):
+        r = run_at(qps, 17 + qps)
+        sweep.append(r)
+        print(f"# qps {qps}: p50 {r['p50_ms']:.3f} ms p99 {r['p99_ms']:.3f} ms, "
+              f"{r['pairs_per_s'] / 1e6:.2f} M pairs/s, {r['launches']} launches", file=sys.stderr,
+              flush=True)
+    main_r = next(r for r in sweep if r["qps"] == args.qps)
+    achieved = k_bytes / (k_ms * 1e-3) / 1e9
+    line = {
+        "metric": "GJK+EPA pairs/s vs plain FP64 GPU path at 1e-5 rel err; p50/p99 query latency",
+        "value": main_r["pairs_per_s"], "unit": "pairs/s", "n_gpus": 1, "steps": n_req,
+        "warmup": 0, "ms_per_step": main_r["wall_s"] / n_req * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "serving: 20000-body scene (8-512 vertices log-uniform), Poisson "
+                               f"requests at {args.qps} QPS, K ~ U{{1..10000}} pairs, fresh pose per "
+                               "pair side, t ~ U[-10,10]^3", "requests": n_req,
+                   "pairs": total_pairs, "lanes_per_pair": {"gjk": 16, "epa": 32},
+                   "server": {"slots": main_r["slots"], "max_batch_pairs": 65536}},
+        "latency_ms": {"p50": main_r["p50_ms"], "p99": main_r["p99_ms"], "max": main_r["max_ms"]},
+        "qps_sweep": sweep,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_gjk + k_epa_list over 10000 posed pairs (one launch)",
+                     "kernel_ms_mean": k_ms, "algorithmic_bytes_per_launch": k_bytes,
+                     "algorithmic_bytes": "compulsory: vertices once + pair, poses, record, results"},
+    }
+    if not args.no_cpu:
+        base, parity = serving_cpu_baseline(scene, b, reqs)
+        line["cpu_baseline"] = base
+        line["parity"] = parity
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def serving_cpu_baseline(scene, b, reqs, max_pairs: int = 200_000):
+    """The FP64 oracle on the world vertices of the first requests' pairs (up
+    to max_pairs), timed on this box's cores; the server's FP64 records for
+    the same requests are compared with it."""
+    import oracle
+    import ogjk
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    take, n = [], 0
+    for p, q in reqs:
+        if n >= max_pairs:
+            break
+        take.append((p, q))
+        n += len(p)
+    pairs = np.concatenate([p for p, _ in take])
+    poses = np.concatenate([q for _, q in take])
+    pool, off, cnt, pa, pb = oracle.posed_pool(b.coords, b.off, b.cnt, pairs, poses)
+    t = time.perf_counter()
+    ref = oracle.run(3, pool, off, cnt, pa, pb, bits=64, nthreads=cores)
+    el = time.perf_counter() - t
+    srv = ogjk.Server(scene, max_batch_pairs=65536, slots=2)
+    try:
+        outs = [srv.wait(srv.submit(p, q, f64=True))[0] for p, q in take]
+    finally:
+        srv.close()
+    rec = np.concatenate([o["f64"] for o in outs])
+    d, dr = rec[:, 0], ref.distances
+    exact = (d == dr) & (rec[:, 1:4] == ref.simplices["witnesses"][:, 0]).all(1) & \
+            (rec[:, 4:7] == ref.simplices["witnesses"][:, 1]).all(1) & (rec[:, 7:10] == ref.normals).all(1)
+    rel = np.abs(d - dr) / np.maximum(np.abs(dr), 1e-3)
+    parity = {"pairs_checked": int(len(d)), "max_rel_err_distance": float(np.nanmax(rel)),
+              "bit_exact_fraction": float(exact.mean()),
+              "overlap_fraction": float((dr <= EPS64).mean())}
+    base = {"value": len(d) / el, "unit": "pairs/s", "cores": cores, "kind": "port",
+            "sample": f"pairs of the first {len(take)} requests ({len(d)} pairs), FP64 oracle on "
+                      f"their FP64 world vertices, {cores} threads, {el:.2f}s (transform untimed)"}
+    return base, parity
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="sphere64", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="sphere64", choices=sorted(WORKLOADS) + ["serving"])
+    ap.add_argument("--qps", type=int, default=1000, help="serving: request rate")
+    ap.add_argument("--requests", type=int, default=2000, help="serving: requests per rate")
     ap.add_argument("--pairs", type=int, default=0, help="override the pair count")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-FP64 comparison")
@@ -168,6 +318,8 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         return 1
+    if args.workload == "serving":
+        return serving(args)
     from ogjk import autotune
     spec = WORKLOADS[args.workload]
     lg, le = (int(x) for x in args.lanes.split(",")) if args.lanes else spec["lanes"]

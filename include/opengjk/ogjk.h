@@ -153,6 +153,48 @@ ogjk_status ogjk_graph_capture_scene(int mode, const ogjk_scene* scene, const in
 ogjk_status ogjk_graph_launch(ogjk_graph graph, void* stream);
 ogjk_status ogjk_graph_destroy(ogjk_graph graph);
 
+/* Scene bodies under per-pair rigid poses: pair i is (d_pairs[2i],
+ * d_pairs[2i+1]) placed by d_poses[14i..14i+6] and d_poses[14i+7..14i+13],
+ * each pose = unit quaternion (w, x, y, z) then translation (x, y, z); world
+ * vertex = R(q) x + t in FP64.  Results are in world coordinates and equal a
+ * run of the reference on the FP64 world vertices.  Asynchronous on stream;
+ * d_workspace as for ogjk_collide_scene_ws.  Clusters are not used.
+ * Replaces a transform pass (visualization/integrate_final_gjk.cu:102-118,
+ * 304-332) followed by GJK::GPU::computeGJKAndEPA (examples/gpu/example.h:38-43). */
+ogjk_status ogjk_collide_posed(int mode, const ogjk_scene* scene, const int32_t* d_pairs,
+                               const float* d_poses, int32_t n_pairs, gkSimplex* d_simplices,
+                               float* d_distances, float* d_normals, double* d_f64,
+                               void* d_workspace, size_t workspace_bytes, void* stream);
+
+/* Serving loop over a resident scene (no counterpart in the reference, whose
+ * calls are synchronous and allocate per call, GJK/gpu/openGJK.cu:2791-2883).
+ * Requests of posed pairs (layout of ogjk_collide_posed, host memory) are
+ * admitted into up to `slots` in-flight launches of at most max_batch_pairs
+ * pairs each (fewer slots if memory_budget bytes of device memory cannot hold
+ * them), batched in arrival order as slots free up, and answered into the
+ * caller's host buffers (simplices, normals and f64 may be NULL).  Buffers
+ * must stay valid until ogjk_server_wait returns for the ticket.  Latency is
+ * submit to results in the caller's buffers. */
+typedef struct ogjk_server_config {
+  int32_t max_batch_pairs; /* pairs per launch and per request (default 65536) */
+  int32_t slots;           /* launches in flight (default 4) */
+  int64_t memory_budget;   /* device bytes for slots, 0: no limit */
+} ogjk_server_config;
+typedef struct ogjk_server_s* ogjk_server;
+ogjk_status ogjk_server_create(const ogjk_scene* scene, const ogjk_server_config* config,
+                               ogjk_server* server);
+ogjk_status ogjk_server_submit(ogjk_server server, const int32_t* pairs, const float* poses,
+                               int32_t n_pairs, gkSimplex* simplices, float* distances,
+                               float* normals, double* f64, int64_t* ticket);
+/* Blocks until the request is answered; latency_s and batch (the launch that
+ * carried it) may be NULL.  Each ticket is waited for once. */
+ogjk_status ogjk_server_wait(ogjk_server server, int64_t ticket, double* latency_s,
+                             int64_t* batch);
+ogjk_status ogjk_server_poll(ogjk_server server, int64_t ticket, int* done);
+ogjk_status ogjk_server_info(ogjk_server server, int32_t* slots, int32_t* capacity,
+                             int64_t* launches, int64_t* pairs_done);
+ogjk_status ogjk_server_destroy(ogjk_server server);
+
 /* Slots a cluster build needs for a scene of n_verts vertices in n_bodies
  * bodies (padding to 32 per body included). */
 int64_t ogjk_cluster_slots(int64_t n_verts, int32_t n_bodies);

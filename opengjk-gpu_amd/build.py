@@ -31,7 +31,8 @@ DEVICE_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-LIB_SOURCES = ["ogjk_kernels.hip", "ogjk_cluster.hip", "ogjk_api.hip", "ogjk_dropin.hip"]
+LIB_SOURCES = ["ogjk_kernels.hip", "ogjk_cluster.hip", "ogjk_api.hip", "ogjk_serve.hip",
+               "ogjk_dropin.hip"]
 
 
 def _run(cmd, cwd=None):

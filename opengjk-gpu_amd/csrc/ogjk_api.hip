@@ -24,6 +24,19 @@ ogjk_status fail(ogjk_status st, const char* fmt, ...) {
   return st;
 }
 
+}  // namespace
+
+// Records the thread's error message; shared with the serving runtime.
+ogjk_status ogjk_fail(ogjk_status st, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return st;
+}
+
+namespace {
+
 ogjk_status hip_fail(hipError_t e, const char* what) {
   return fail(e == hipErrorOutOfMemory ? OGJK_ERR_NOMEM : OGJK_ERR_HIP, "%s: %s", what,
               hipGetErrorString(e));
@@ -297,6 +310,25 @@ ogjk_status ogjk_collide_scene_ws(int mode, const ogjk_scene* scene, const int32
   if (s2 != OGJK_OK) return s2;
   const ogjk::OutArgs oa{d_simplices, d_distances, d_normals, d_f64};
   OGJK_HIP(ogjk::launch_scene(mode, sa, oa, n_pairs, d_workspace, (hipStream_t)stream),
+           "kernel launch");
+  return OGJK_OK;
+}
+
+ogjk_status ogjk_collide_posed(int mode, const ogjk_scene* scene, const int32_t* d_pairs,
+                               const float* d_poses, int32_t n_pairs, gkSimplex* d_simplices,
+                               float* d_distances, float* d_normals, double* d_f64,
+                               void* d_workspace, size_t workspace_bytes, void* stream) {
+  const ogjk_status st = check_out(mode, n_pairs, d_simplices, d_distances, d_normals);
+  if (st != OGJK_OK) return st;
+  if (n_pairs == 0) return OGJK_OK;
+  if (!d_pairs || !d_poses) return fail(OGJK_ERR_INVALID, "posed pairs need pairs and poses");
+  if (mode != OGJK_MODE_EPA && (!d_workspace || workspace_bytes < ogjk::workspace_bytes(n_pairs)))
+    return fail(OGJK_ERR_INVALID, "workspace of %zu bytes needed", ogjk::workspace_bytes(n_pairs));
+  ogjk::SceneArgs sa;
+  const ogjk_status s2 = scene_args(scene, d_pairs, n_pairs, &sa);
+  if (s2 != OGJK_OK) return s2;
+  const ogjk::OutArgs oa{d_simplices, d_distances, d_normals, d_f64};
+  OGJK_HIP(ogjk::launch_posed(mode, sa, d_poses, oa, n_pairs, d_workspace, (hipStream_t)stream),
            "kernel launch");
   return OGJK_OK;
 }

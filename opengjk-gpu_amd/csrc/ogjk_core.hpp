@@ -353,6 +353,94 @@ OGJK_DEV Sup support(const CBody& A, const Query& q, int lane) {
   return finish1<LPP, false>(c, q);
 }
 
+// ------------------------------------------------------------ posed bodies
+//
+// A body placed by a rigid pose: world vertex = R x + t, R from a unit
+// quaternion (w, x, y, z) and t stored as 7 floats.  World coordinates are
+// computed in FP64, each component as ((R0 x + R1 y) + R2 z) + t, the same
+// arithmetic the checker uses to build its world vertices.  A support query
+// sweeps the FP32 local vertices along R^T d and re-ranks its candidates on
+// their FP64 world coordinates, so the selection is that of a sweep over the
+// FP64 world vertices.
+
+struct Pose {
+  double r[9];
+  double t[3];
+  OGJK_DEV static Pose load(const float* p) {
+    const double w = p[0], x = p[1], y = p[2], z = p[3];
+    Pose P;
+    P.r[0] = 1.0 - 2.0 * (y * y + z * z);
+    P.r[1] = 2.0 * (x * y - w * z);
+    P.r[2] = 2.0 * (x * z + w * y);
+    P.r[3] = 2.0 * (x * y + w * z);
+    P.r[4] = 1.0 - 2.0 * (x * x + z * z);
+    P.r[5] = 2.0 * (y * z - w * x);
+    P.r[6] = 2.0 * (x * z - w * y);
+    P.r[7] = 2.0 * (y * z + w * x);
+    P.r[8] = 1.0 - 2.0 * (x * x + y * y);
+    P.t[0] = p[4];
+    P.t[1] = p[5];
+    P.t[2] = p[6];
+    return P;
+  }
+  OGJK_DEV D3 world(double x, double y, double z) const {
+    return D3{((r[0] * x + r[1] * y) + r[2] * z) + t[0], ((r[3] * x + r[4] * y) + r[5] * z) + t[1],
+              ((r[6] * x + r[7] * y) + r[8] * z) + t[2]};
+  }
+  OGJK_DEV D3 to_local(D3 d) const {  // R^T d
+    return D3{(r[0] * d.x + r[3] * d.y) + r[6] * d.z, (r[1] * d.x + r[4] * d.y) + r[7] * d.z,
+              (r[2] * d.x + r[5] * d.y) + r[8] * d.z};
+  }
+};
+
+struct PBody : Body<float, 1> {
+  const float* pose;  // 7 floats: quaternion w, x, y, z, then translation
+  OGJK_DEV D3 d(int i) const { return Pose::load(pose).world(rx(i), ry(i), rz(i)); }
+  OGJK_DEV double c(int i, int k) const {
+    const D3 p = d(i);
+    return k == 0 ? p.x : (k == 1 ? p.y : p.z);
+  }
+};
+
+// Local FP32 sweep along R^T d.  Besides the FP32 error (16u r1 in scaled
+// units), each candidate's FP64 world value may differ from the real
+// (R x + t).d by at most u64 (45 r1 + 15 |t|_1) scaled, < 2^-46 (r1 + |t|_1):
+// the window widens by twice that.
+template <int LPP>
+OGJK_DEV Sup support(const PBody& A, const Query& q, int lane) {
+  const Pose P = Pose::load(A.pose);
+  Sweep32 sw = sweep32_begin(P.to_local(q.dir));
+  for (int i = lane; i < A.n; i += LPP) sweep32_step(A, i, sw);
+  const float m = group_max_f<LPP>(sw.best);
+  const float t1 = (float)(fabs(P.t[0]) + fabs(P.t[1]) + fabs(P.t[2]));
+  const float T = m - (0x1.0p-19f * A.r1 + 0x1.0p-45f * (A.r1 + t1));
+  const bool multi = group_any<LPP>(sw.second >= T);
+  Cand c{-INFINITY, 0x7fffffff, D3{0.0, 0.0, 0.0}};
+  if (!multi) {
+    if (sw.bi >= 0 && sw.best >= T) {
+      c.p = P.world(sw.bx, sw.by, sw.bz);
+      c.v = dot(c.p, q.dir);
+      c.i = sw.bi;
+    }
+  } else {
+    for (int i = lane; i < A.n; i += LPP) {
+      const float px = A.rx(i), py = A.ry(i), pz = A.rz(i);
+      const float d = fmaf(px, sw.dx, fmaf(py, sw.dy, pz * sw.dz));
+      if (d >= T) {
+        const D3 p = P.world(px, py, pz);
+        const double v = dot(p, q.dir);
+        if (v > c.v) {  // ascending i within the lane keeps the lowest index
+          c.v = v;
+          c.i = i;
+          c.p = p;
+        }
+      }
+    }
+  }
+  if (!(c.v == c.v)) c.v = -INFINITY;
+  return finish1<LPP>(c, q);
+}
+
 // Field-wise value selects (a select between two structs' addresses would
 // put them in scratch memory).
 template <class T, int STRIDE>
@@ -363,6 +451,10 @@ OGJK_DEV Body<T, STRIDE> pick(bool c, const Body<T, STRIDE>& a, const Body<T, ST
 OGJK_DEV CBody pick(bool c, const CBody& a, const CBody& b) {
   return CBody{pick(c, static_cast<const Body<float, 1>&>(a), static_cast<const Body<float, 1>&>(b)),
                a.cv, c ? a.slot : b.slot};
+}
+OGJK_DEV PBody pick(bool c, const PBody& a, const PBody& b) {
+  return PBody{pick(c, static_cast<const Body<float, 1>&>(a), static_cast<const Body<float, 1>&>(b)),
+               c ? a.pose : b.pose};
 }
 OGJK_DEV Query pick(bool c, const Query& a, const Query& b) {
   return Query{sel(c, a.dir, b.dir), c ? a.has_inc : b.has_inc, c ? a.inc_idx : b.inc_idx,
@@ -807,27 +899,27 @@ struct Wit {
   D3 w0, w1;
 };
 
-template <class T, int STRIDE>
-OGJK_DEV void W0D(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, const Simplex& s, Wit& w) {
+template <class BT>
+OGJK_DEV void W0D(const BT& A, const BT& B, const Simplex& s, Wit& w) {
   w.w0 = A.d(s.ia[0]);
   w.w1 = B.d(s.ib[0]);
 }
 
 // Blend of original vertices: sum_k (coord_k * a_k), left to right.
-template <class T, int STRIDE>
-OGJK_DEV double blend_c(const Body<T, STRIDE>& b, const int* idx, const double* a, int m, int c) {
+template <class BT>
+OGJK_DEV double blend_c(const BT& b, const int* idx, const double* a, int m, int c) {
   double acc = 0.0;
   for (int k = 0; k < m; k++) {
     const int i = idx[k];
-    const double x = (double)(c == 0 ? b.rx(i) : (c == 1 ? b.ry(i) : b.rz(i)));
+    const double x = b.c(i, c);
     acc = (k == 0) ? x * a[k] : acc + x * a[k];
   }
   return acc;
 }
 
 // W1D: GJK/cpu/openGJK.c:668-700 (det == 0 falls through, as the reference)
-template <class T, int STRIDE>
-OGJK_DEV void W1D(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, const Simplex& s, Wit& w) {
+template <class BT>
+OGJK_DEV void W1D(const BT& A, const BT& B, const Simplex& s, Wit& w) {
   const D3 p = s.p[0], q = s.p[1];
   const D3 pq = sub(q, p), po = neg(p);
   const double det = dot(pq, pq);
@@ -840,8 +932,8 @@ OGJK_DEV void W1D(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, const Simp
 }
 
 // W2D: GJK/cpu/openGJK.c:702-795
-template <class T, int STRIDE>
-OGJK_DEV void W2D(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, Simplex& s, Wit& w) {
+template <class BT>
+OGJK_DEV void W2D(const BT& A, const BT& B, Simplex& s, Wit& w) {
   const D3 p = s.p[0], q = s.p[1], r = s.p[2];
   const D3 pq = sub(q, p), pr = sub(r, p), po = neg(p);
   const double T00 = dot(pq, pq), T01 = dot(pq, pr), T11 = dot(pr, pr);
@@ -871,8 +963,8 @@ OGJK_DEV void W2D(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, Simplex& s
 }
 
 // W3D: GJK/cpu/openGJK.c:797-929
-template <class T, int STRIDE>
-OGJK_DEV void W3D(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, Simplex& s, Wit& w) {
+template <class BT>
+OGJK_DEV void W3D(const BT& A, const BT& B, Simplex& s, Wit& w) {
   const D3 p = s.p[0], q = s.p[1], r = s.p[2], t = s.p[3];
   const D3 pq = sub(q, p), pr = sub(r, p), ps = sub(t, p), po = neg(p);
   const double T00 = dot(pq, pq), T01 = dot(pq, pr), T02 = dot(pq, ps);
@@ -916,8 +1008,8 @@ OGJK_DEV void W3D(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, Simplex& s
 }
 
 // compute_witnesses: GJK/cpu/openGJK.c:931-950
-template <class T, int STRIDE>
-OGJK_DEV void witnesses(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, Simplex& s, Wit& w) {
+template <class BT>
+OGJK_DEV void witnesses(const BT& A, const BT& B, Simplex& s, Wit& w) {
   if (s.n == 4)
     W3D(A, B, s, w);
   else if (s.n == 3)

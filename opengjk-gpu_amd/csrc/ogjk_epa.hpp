@@ -225,8 +225,8 @@ OGJK_DEV bool is_new_point(const Simplex& s, D3 p) {
   return fresh;
 }
 
-template <class T, int STRIDE>
-OGJK_DEV void zero_depth(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, int ia, int ib,
+template <class BT>
+OGJK_DEV void zero_depth(const BT& A, const BT& B, int ia, int ib,
                          double& distance, Wit& w, D3& normal) {
   distance = 0.0;
   w.w0 = A.d(ia);
@@ -234,8 +234,8 @@ OGJK_DEV void zero_depth(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, int
   normal = contact_normal(w.w0, w.w1);
 }
 
-template <class T, int STRIDE>
-OGJK_DEV void face_result(const Body<T, STRIDE>& A, const Body<T, STRIDE>& B, const EpaShared& sh,
+template <class BT>
+OGJK_DEV void face_result(const BT& A, const BT& B, const EpaShared& sh,
                           int verts, D3 n, double cdist, double& distance, Wit& w, D3& normal) {
   const int v0 = verts & 0xff, v1 = (verts >> 8) & 0xff, v2 = (verts >> 16) & 0xff;
   double a[3];

@@ -80,6 +80,10 @@ hipError_t launch_scene_f64(int mode, const SceneArgs& s, const OutArgs& o, int 
                             hipStream_t st);
 hipError_t launch_poly(int mode, const PolyArgs& p, const OutArgs& o, int n, void* ws,
                        hipStream_t st);
+// Scene bodies under per-pair poses (14 floats per pair: quaternion w, x, y,
+// z and translation of the first body, then of the second); s.pairs required.
+hipError_t launch_posed(int mode, const SceneArgs& s, const float* poses, const OutArgs& o,
+                        int n, void* ws, hipStream_t st);
 // Vertex clusters (see ogjk_cluster.hip): slots needed, and the build.
 int64_t cluster_slots(int64_t n_verts, int32_t n_bodies);
 hipError_t build_clusters(const float* x, const float* y, const float* z, const int64_t* off,

@@ -68,6 +68,37 @@ struct SceneSrc {
   }
 };
 
+// Scene bodies under per-pair rigid poses: pair i = (pairs[2i], pairs[2i+1])
+// placed by poses[14i .. 14i+6] and poses[14i+7 .. 14i+13].
+struct PoseSrc {
+  const float* x;
+  const float* y;
+  const float* z;
+  const int64_t* off;
+  const int32_t* cnt;
+  const float* r1;
+  int32_t nb;
+  const int32_t* pairs;
+  const float* poses;
+  using BodyT = PBody;
+
+  OGJK_DEV int pair_size(int i) const {
+    const int a = pairs[2 * i], b = pairs[2 * i + 1];
+    if (a < 0 || a >= nb || b < 0 || b >= nb) return 0;
+    return cnt[a] > cnt[b] ? cnt[a] : cnt[b];
+  }
+
+  template <int LPP>
+  OGJK_DEV bool load(int i, int lane, BodyT& A, BodyT& B) const {
+    const int a = pairs[2 * i], b = pairs[2 * i + 1];
+    if (a < 0 || a >= nb || b < 0 || b >= nb) return false;
+    const int64_t oa = off[a], ob = off[b];
+    A = PBody{{x + oa, y + oa, z + oa, cnt[a], r1[a]}, poses + 14 * (int64_t)i};
+    B = PBody{{x + ob, y + ob, z + ob, cnt[b], r1[b]}, poses + 14 * (int64_t)i + 7};
+    return true;
+  }
+};
+
 struct PolySrc {
   const gkPolytope* a;
   const gkPolytope* b;
@@ -174,9 +205,9 @@ OGJK_DEV void store(const OutArgs& o, int i, const Simplex& s, const Wit& w, dou
 // are exact differences A[ia] - B[ib]; the FP32 record rounds them, so each
 // one whose rounding matches is rebuilt in FP64 from its indices (any other
 // value is taken as given).  Out-of-range indices are clamped to vertex 0.
-template <class T, int STRIDE>
-OGJK_DEV void load_input(const OutArgs& o, int i, const Body<T, STRIDE>& A,
-                         const Body<T, STRIDE>& B, Simplex& s, Wit& w, double& dist) {
+template <class BT>
+OGJK_DEV void load_input(const OutArgs& o, int i, const BT& A,
+                         const BT& B, Simplex& s, Wit& w, double& dist) {
   const gkSimplex* r = o.simplices + i;
   s.n = r->nvrtx;
 #pragma unroll
@@ -729,6 +760,13 @@ hipError_t launch_scene_f64(int mode, const SceneArgs& s, const OutArgs& o, int 
   const SceneSrc<double> src{(const double*)s.x, (const double*)s.y, (const double*)s.z, s.off,
                              s.cnt, nullptr, s.n_bodies, s.pairs, ClusterView{}, nullptr,
                              0x7fffffff};
+  return launch_any(mode, src, o, n, ws, st);
+}
+
+hipError_t launch_posed(int mode, const SceneArgs& s, const float* poses, const OutArgs& o,
+                        int n, void* ws, hipStream_t st) {
+  const PoseSrc src{(const float*)s.x, (const float*)s.y, (const float*)s.z, s.off, s.cnt, s.r1,
+                    s.n_bodies, s.pairs, poses};
   return launch_any(mode, src, o, n, ws, st);
 }
 

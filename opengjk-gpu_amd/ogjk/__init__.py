@@ -39,7 +39,9 @@ C_ABI_SYMBOLS = [
     "ogjk_collide_scene", "ogjk_pack_scene_f64", "ogjk_collide_scene_f64",
     "ogjk_collide_polytopes", "ogjk_collide_polytopes_indexed", "ogjk_collide_host",
     "ogjk_collide_host_indexed", "ogjk_cluster_slots", "ogjk_build_clusters",
-    "ogjk_graph_capture_scene", "ogjk_graph_launch", "ogjk_graph_destroy",
+    "ogjk_graph_capture_scene", "ogjk_graph_launch", "ogjk_graph_destroy", "ogjk_collide_posed",
+    "ogjk_server_create", "ogjk_server_submit", "ogjk_server_wait", "ogjk_server_poll",
+    "ogjk_server_info", "ogjk_server_destroy",
 ]
 
 
@@ -57,6 +59,10 @@ class _Scene(C.Structure):
     _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("body_off", C.c_void_p),
                 ("body_cnt", C.c_void_p), ("body_r1", C.c_void_p), ("n_bodies", C.c_int32),
                 ("clusters", C.POINTER(_Clusters))]
+
+
+class _ServerConfig(C.Structure):
+    _fields_ = [("max_batch_pairs", C.c_int32), ("slots", C.c_int32), ("memory_budget", C.c_int64)]
 
 
 class _Scene64(C.Structure):
@@ -91,6 +97,16 @@ def lib():
             "ogjk_graph_capture_scene": (C.c_int, [C.c_int, C.POINTER(_Scene), vp, i32, vp, vp, vp,
                                                    vp, C.POINTER(vp)]),
             "ogjk_graph_launch": (C.c_int, [vp, vp]),
+            "ogjk_collide_posed": (C.c_int, [C.c_int, C.POINTER(_Scene), vp, vp, i32, vp, vp, vp,
+                                             vp, vp, C.c_size_t, vp]),
+            "ogjk_server_create": (C.c_int, [C.POINTER(_Scene), C.POINTER(_ServerConfig),
+                                             C.POINTER(vp)]),
+            "ogjk_server_submit": (C.c_int, [vp, vp, vp, i32, vp, vp, vp, vp, C.POINTER(i64)]),
+            "ogjk_server_wait": (C.c_int, [vp, i64, C.POINTER(C.c_double), C.POINTER(i64)]),
+            "ogjk_server_poll": (C.c_int, [vp, i64, C.POINTER(C.c_int)]),
+            "ogjk_server_info": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i64),
+                                           C.POINTER(i64)]),
+            "ogjk_server_destroy": (C.c_int, [vp]),
             "ogjk_graph_destroy": (C.c_int, [vp]),
             "ogjk_build_clusters": (C.c_int, [C.POINTER(_Scene), C.POINTER(_Clusters), vp]),
             "ogjk_collide_scene": (C.c_int, [C.c_int, C.POINTER(_Scene), vp, i32, vp, vp, vp, vp, vp]),
@@ -241,6 +257,22 @@ class DeviceScene:
             raise OgjkError("no clusters built")
         self.clusters.min_verts = int(min_verts)
 
+    def collide_posed(self, mode: int, pairs, poses, out=None, f64_record: bool = False,
+                      stream=None):
+        """Scene bodies under per-pair poses (ogjk_collide_posed): pairs (n, 2)
+        int32 and poses (n, 14) float32 device tensors; asynchronous."""
+        n = int(pairs.shape[0])
+        out = out or self.outputs(n, f64_record)
+        need = int(lib().ogjk_workspace_bytes(n))
+        ws = self.torch.empty(max(need, 1), dtype=self.torch.uint8, device=self.x.device)
+        f = out["f64"].data_ptr() if out.get("f64") is not None else None
+        _check(lib().ogjk_collide_posed(mode, C.byref(self.scene), pairs.data_ptr(),
+                                        poses.data_ptr(), n, out["simplices"].data_ptr(),
+                                        out["distances"].data_ptr(), out["normals"].data_ptr(), f,
+                                        ws.data_ptr(), need, self._stream(stream)))
+        out["_workspace"] = ws  # alive until the caller drops the outputs
+        return out
+
     def capture(self, mode: int, n_pairs: int, pairs=None, out=None, f64_record: bool = False):
         """Capture one launch over this scene as a HIP graph (see Graph)."""
         out = out or self.outputs(n_pairs, f64_record)
@@ -284,6 +316,62 @@ class DeviceScene:
                                             out["simplices"].data_ptr(), out["distances"].data_ptr(),
                                             out["normals"].data_ptr(), f, st))
         return out
+
+
+class Server:
+    """Serving loop over a resident scene (ogjk_server_*): requests of posed
+    pairs are batched as they arrive and answered into numpy arrays."""
+
+    def __init__(self, scene, max_batch_pairs: int = 65536, slots: int = 4, memory_budget: int = 0):
+        if scene.scene.clusters:
+            raise OgjkError("serving uses scenes without clusters")
+        self._scene = scene
+        cfg = _ServerConfig(max_batch_pairs, slots, memory_budget)
+        h = C.c_void_p()
+        _check(lib().ogjk_server_create(C.byref(scene.scene), C.byref(cfg), C.byref(h)))
+        self._h = h
+        self._live = {}
+
+    def submit(self, pairs, poses, f64: bool = False) -> int:
+        """pairs: (n, 2) int32 body ids; poses: (n, 14) float32 (quaternion
+        w, x, y, z and translation of each body).  Returns a ticket."""
+        pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        poses = np.ascontiguousarray(poses, dtype=np.float32).reshape(-1, 14)
+        n = len(pairs)
+        out = {"simplices": np.zeros(n, SIMPLEX_DTYPE), "distances": np.zeros(n, np.float32),
+               "normals": np.zeros((n, 3), np.float32),
+               "f64": np.zeros((n, F64_RECORD), np.float64) if f64 else None}
+        t = C.c_int64()
+        _check(lib().ogjk_server_submit(self._h, _ptr(pairs), _ptr(poses), n,
+                                        _ptr(out["simplices"]), _ptr(out["distances"]),
+                                        _ptr(out["normals"]), _ptr(out["f64"]), C.byref(t)))
+        self._live[t.value] = (pairs, poses, out)
+        return t.value
+
+    def wait(self, ticket: int):
+        """Blocks until answered; returns (outputs, latency_s, batch id)."""
+        lat, batch = C.c_double(), C.c_int64()
+        st = lib().ogjk_server_wait(self._h, int(ticket), C.byref(lat), C.byref(batch))
+        _, _, out = self._live.pop(ticket)
+        _check(st)
+        return out, lat.value, batch.value
+
+    def info(self) -> dict:
+        sl, cap, la, pd = C.c_int32(), C.c_int32(), C.c_int64(), C.c_int64()
+        _check(lib().ogjk_server_info(self._h, C.byref(sl), C.byref(cap), C.byref(la), C.byref(pd)))
+        return {"slots": sl.value, "capacity": cap.value, "launches": la.value,
+                "pairs_done": pd.value}
+
+    def close(self) -> None:
+        if self._h is not None and self._h.value:
+            _check(lib().ogjk_server_destroy(self._h))
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Graph:

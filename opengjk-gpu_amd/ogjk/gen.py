@@ -86,3 +86,23 @@ def from_bodies(bodies_a, bodies_b) -> Batch:
     coords = np.concatenate(bodies, axis=0) if bodies else np.zeros((0, 3), np.float32)
     pa = np.arange(0, len(bodies), 2, dtype=np.int32)
     return Batch(np.ascontiguousarray(coords), off, cnt, pa, pa + 1)
+
+
+def posed_requests(n_bodies: int, n_requests: int, seed: int = 4, kmin: int = 1,
+                   kmax: int = 10_000, tbox: float = 10.0):
+    """Serving requests over a scene of n_bodies: each has K ~ U{kmin..kmax}
+    pairs (i, j), i != j uniform, and a fresh pose per pair side: unit
+    quaternion (w, x, y, z) = normalised 4 x N(0,1), translation U[-tbox,
+    tbox]^3.  Returns a list of (pairs (K, 2) int32, poses (K, 14) float32)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_requests):
+        k = int(rng.integers(kmin, kmax + 1))
+        i = rng.integers(0, n_bodies, size=k)
+        j = (i + rng.integers(1, n_bodies, size=k)) % n_bodies
+        q = rng.normal(size=(k, 2, 4))
+        q /= np.linalg.norm(q, axis=2, keepdims=True)
+        t = rng.uniform(-tbox, tbox, size=(k, 2, 3))
+        poses = np.concatenate([q, t], axis=2).reshape(k, 14).astype(np.float32)
+        out.append((np.stack([i, j], 1).astype(np.int32), poses))
+    return out

@@ -159,3 +159,53 @@ def ref_run(mode: int, coords, off, cnt, pa, pb) -> Result:
         dist[i] = d.value
         out[i] = np.frombuffer(bytes(s), dtype=sd)[0]
     return Result(dist, out, normals, None)
+
+
+# ------------------------------------------------------------- posed bodies
+
+def pose_matrix(q):
+    """FP64 rotation of a float32 quaternion (w, x, y, z), entry by entry in
+    the arithmetic the posed GPU path uses."""
+    w, x, y, z = (np.float64(v) for v in q)
+    return np.array([[1.0 - 2.0 * (y * y + z * z), 2.0 * (x * y - w * z), 2.0 * (x * z + w * y)],
+                     [2.0 * (x * y + w * z), 1.0 - 2.0 * (x * x + z * z), 2.0 * (y * z - w * x)],
+                     [2.0 * (x * z - w * y), 2.0 * (y * z + w * x), 1.0 - 2.0 * (x * x + y * y)]])
+
+
+def world_vertices(local, pose):
+    """FP64 world vertices R x + t of an (m, 3) float32 body under a pose of
+    7 float32 (quaternion, translation): ((r0 x + r1 y) + r2 z) + t."""
+    R = pose_matrix(pose[:4])
+    t = pose[4:7].astype(np.float64)
+    p = np.asarray(local, dtype=np.float64)
+    px, py, pz = p[:, 0], p[:, 1], p[:, 2]
+    return np.stack([((R[k, 0] * px + R[k, 1] * py) + R[k, 2] * pz) + t[k] for k in range(3)], 1)
+
+
+def posed_pool(coords, off, cnt, pairs, poses):
+    """World-vertex pool for posed pairs: body 2i / 2i+1 = pair i's bodies
+    under its two poses, computed in the arithmetic of world_vertices.
+    Returns (coords f64, off, cnt, pa, pb)."""
+    n = len(pairs)
+    bodies = np.asarray(pairs, dtype=np.int64).reshape(-1)        # 2n sides
+    side_pose = np.asarray(poses, dtype=np.float32).reshape(2 * n, 7)
+    cnts = np.asarray(cnt, dtype=np.int64)[bodies]
+    offs = np.zeros(2 * n, np.int64)
+    if n:
+        offs[1:] = np.cumsum(cnts)[:-1]
+    total = int(cnts.sum())
+    side = np.repeat(np.arange(2 * n), cnts)                       # side of each world vertex
+    src = np.repeat(np.asarray(off, dtype=np.int64)[bodies] - offs, cnts) + np.arange(total)
+    p = np.asarray(coords, dtype=np.float64)[src]
+    q = side_pose.astype(np.float64)
+    w, x, y, z = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    R = [1.0 - 2.0 * (y * y + z * z), 2.0 * (x * y - w * z), 2.0 * (x * z + w * y),
+         2.0 * (x * y + w * z), 1.0 - 2.0 * (x * x + z * z), 2.0 * (y * z - w * x),
+         2.0 * (x * z - w * y), 2.0 * (y * z + w * x), 1.0 - 2.0 * (x * x + y * y)]
+    R = [r[side] for r in R]
+    t = [q[side, 4], q[side, 5], q[side, 6]]
+    px, py, pz = p[:, 0], p[:, 1], p[:, 2]
+    pool = np.stack([((R[3 * k] * px + R[3 * k + 1] * py) + R[3 * k + 2] * pz) + t[k]
+                     for k in range(3)], 1)
+    return (pool, offs, cnts.astype(np.int32), np.arange(0, 2 * n, 2, dtype=np.int32),
+            np.arange(1, 2 * n, 2, dtype=np.int32))

@@ -147,3 +147,41 @@ def test_oracle64_matches_oracle32_within_fp32_error():
     sep = (r64.distances > 1e-2) & (r32.distances > 1e-2)
     # FP32 reference tolerances give ~1e-3 relative (openGJK.c:41); FP64 is the checker.
     assert np.allclose(r32.distances[sep], r64.distances[sep], rtol=3e-3)
+
+
+# ------------------------------------------------------------ posed bodies
+
+def test_world_vertices_closed_forms():
+    """The checker's world transform: identity and half-turn quaternions give
+    exact closed forms; the vectorised pool equals the per-body transform."""
+    rng = np.random.default_rng(3)
+    local = rng.normal(size=(50, 3)).astype(np.float32)
+    ident = np.array([1, 0, 0, 0, 1.5, -2.0, 0.25], np.float32)
+    assert np.array_equal(oracle.world_vertices(local, ident),
+                          local.astype(np.float64) + np.array([1.5, -2.0, 0.25]))
+    half_z = np.array([0, 0, 0, 1, 0, 0, 0], np.float32)          # 180 degrees about z
+    w = oracle.world_vertices(local, half_z)
+    assert np.array_equal(w, local.astype(np.float64) * np.array([-1.0, -1.0, 1.0]))
+    b = gen.make(gen.ELLIPSOID, 40, seed=5, box=0.0, nmin=8, nmax=200)
+    pairs, poses = gen.posed_requests(len(b.cnt), 1, seed=6, kmin=30, kmax=30)[0]
+    pool, off, cnt, pa, pb = oracle.posed_pool(b.coords, b.off, b.cnt, pairs, poses)
+    for i in range(30):
+        for s in range(2):
+            k = pairs[i, s]
+            ref = oracle.world_vertices(b.coords[b.off[k]:b.off[k] + b.cnt[k]], poses[i, 7 * s:7 * s + 7])
+            assert np.array_equal(ref, pool[off[2 * i + s]:off[2 * i + s] + cnt[2 * i + s]])
+
+
+def test_posed_rigid_invariance():
+    """The same rigid motion applied to both bodies leaves distance and depth
+    unchanged (up to rounding of the transformed coordinates)."""
+    b = gen.make(gen.ELLIPSOID, 200, seed=8, box=0.0, nmin=8, nmax=300)
+    pairs, poses = gen.posed_requests(len(b.cnt), 1, seed=9, kmin=200, kmax=200, tbox=1.5)[0]
+    base = oracle.run(oracle.GJK_EPA, *oracle.posed_pool(b.coords, b.off, b.cnt, pairs, poses))
+    # compose every pose with one extra rotation + translation, in FP64 on the
+    # world pool directly
+    pool, off, cnt, pa, pb = oracle.posed_pool(b.coords, b.off, b.cnt, pairs, poses)
+    R = oracle.pose_matrix(np.array([0.5, 0.5, -0.5, 0.5], np.float32))
+    moved = pool @ R.T + np.array([3.0, -1.0, 2.0])
+    res = oracle.run(oracle.GJK_EPA, moved, off, cnt, pa, pb)
+    assert np.allclose(res.distances, base.distances, rtol=1e-9, atol=1e-9)
