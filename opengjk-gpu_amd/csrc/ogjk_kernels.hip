@@ -11,6 +11,10 @@
 #ifndef OGJK_GJK_WAVES
 #define OGJK_GJK_WAVES 3
 #endif
+// Clustered and posed bodies carry more live state through GJK.
+#ifndef OGJK_GJK_WAVES_WIDE
+#define OGJK_GJK_WAVES_WIDE 3
+#endif
 #ifndef OGJK_EPA_WAVES
 #define OGJK_EPA_WAVES 2
 #endif
@@ -35,6 +39,7 @@ struct SceneSrc {
   const int64_t* cslot;  // first cluster slot of each body
   int32_t cluster_min;   // bodies with at least this many vertices are culled
   using BodyT = std::conditional_t<CL, CBody, Body<T, 1>>;
+  static constexpr int kGjkWaves = CL ? OGJK_GJK_WAVES_WIDE : OGJK_GJK_WAVES;
 
   // max(nA, nB) of pair i, or 0 for an invalid pair (it goes to bucket 0)
   OGJK_DEV int pair_size(int i) const {
@@ -81,6 +86,7 @@ struct PoseSrc {
   const int32_t* pairs;
   const float* poses;
   using BodyT = PBody;
+  static constexpr int kGjkWaves = OGJK_GJK_WAVES_WIDE;
 
   OGJK_DEV int pair_size(int i) const {
     const int a = pairs[2 * i], b = pairs[2 * i + 1];
@@ -105,6 +111,7 @@ struct PolySrc {
   const gkCollisionPair* pairs;
   int32_t npool;
   using BodyT = Body<float, 3>;
+  static constexpr int kGjkWaves = OGJK_GJK_WAVES;
 
   OGJK_DEV int pair_size(int i) const {
     const gkPolytope *pa, *pb;
@@ -482,7 +489,7 @@ __global__ void __launch_bounds__(256) k_scatter(Src src, int n, BucketPlan plan
 // its EPA class's job list with its FP64 state.  With a single bucket the
 // list is the identity and is not materialised.
 template <int LPP, bool EMIT, class Src>
-__global__ void __launch_bounds__(64, OGJK_GJK_WAVES)
+__global__ void __launch_bounds__(64, Src::kGjkWaves)
     k_gjk(Src src, OutArgs out, int n, BucketPlan plan, int bucket, unsigned* c, const int* list,
           EpaJob* jobs) {
   constexpr int GPB = 64 / LPP;
