@@ -40,6 +40,8 @@ struct SceneSrc {
   int32_t cluster_min;   // bodies with at least this many vertices are culled
   using BodyT = std::conditional_t<CL, CBody, Body<T, 1>>;
   static constexpr int kGjkWaves = CL ? OGJK_GJK_WAVES_WIDE : OGJK_GJK_WAVES;
+  // GJK stages small pairs in LDS (float scenes without clusters)
+  static constexpr bool kStage = std::is_same_v<T, float> && !CL;
 
   // max(nA, nB) of pair i, or 0 for an invalid pair (it goes to bucket 0)
   OGJK_DEV int pair_size(int i) const {
@@ -87,6 +89,7 @@ struct PoseSrc {
   const float* poses;
   using BodyT = PBody;
   static constexpr int kGjkWaves = OGJK_GJK_WAVES_WIDE;
+  static constexpr bool kStage = false;
 
   OGJK_DEV int pair_size(int i) const {
     const int a = pairs[2 * i], b = pairs[2 * i + 1];
@@ -112,6 +115,7 @@ struct PolySrc {
   int32_t npool;
   using BodyT = Body<float, 3>;
   static constexpr int kGjkWaves = OGJK_GJK_WAVES;
+  static constexpr bool kStage = false;
 
   OGJK_DEV int pair_size(int i) const {
     const gkPolytope *pa, *pb;
@@ -389,6 +393,40 @@ OGJK_DEV unsigned claim(unsigned* c, int queue, unsigned count, int& slice, int&
   return result;
 }
 
+// LDS staging for GJK: a pair whose two bodies hold at most CAP vertices
+// together is copied once into the group's LDS slice (x | y | z, CAP floats
+// each) and its bodies are pointed there, so the ~10 support sweeps of its
+// GJK read LDS instead of L2 (11% faster on 64-vertex pairs; EPA, bound by its
+// reductions rather than its loads, gains nothing).  Group-uniform decision.
+constexpr int kStageGjk = 128;
+
+template <int LPP, int CAP, class BT>
+OGJK_DEV void stage_pair(BT& A, BT& B, float* lds, int lane) {
+  wave_sync();
+  if (A.n + B.n <= CAP) {
+    float* lx = lds;
+    float* ly = lds + CAP;
+    float* lz = lds + 2 * CAP;
+    for (int i = lane; i < A.n; i += LPP) {
+      lx[i] = A.x[i];
+      ly[i] = A.y[i];
+      lz[i] = A.z[i];
+    }
+    for (int i = lane; i < B.n; i += LPP) {
+      lx[A.n + i] = B.x[i];
+      ly[A.n + i] = B.y[i];
+      lz[A.n + i] = B.z[i];
+    }
+    A.x = lx;
+    A.y = ly;
+    A.z = lz;
+    B.x = lx + A.n;
+    B.y = ly + A.n;
+    B.z = lz + A.n;
+  }
+  wave_sync();
+}
+
 // Stable partition of the pairs by bucket, in three kernels: per-block
 // counts, one exclusive scan over blocks, scatter.  The list keeps the input
 // order inside each bucket, so concurrent waves work on neighbouring pairs
@@ -507,6 +545,10 @@ __global__ void __launch_bounds__(64, Src::kGjkWaves)
   if (!ok) {
     if (lane == 0) store_invalid(out, pair, EMIT);
     return;
+  }
+  if constexpr (Src::kStage) {
+    __shared__ float stg[GPB][3 * kStageGjk];
+    stage_pair<LPP, kStageGjk>(A, B, stg[threadIdx.x / LPP], lane);
   }
   Simplex s = {};
   Wit w = {};

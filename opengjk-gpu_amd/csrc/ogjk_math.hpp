@@ -10,6 +10,7 @@
 
 #pragma clang fp contract(off)
 
+
 namespace ogjk {
 
 #define OGJK_DEV __device__ __forceinline__
@@ -66,8 +67,8 @@ OGJK_DEV int group_prefix(uint64_t gb) {
 // 1 and 2 are quad permutes, 4 and 8 are the row half-mirror and mirror
 // (lane i <-> 7-i, 15-i): applied in increasing m after the lower steps have
 // made each block of m lanes uniform they pair every block with its partner
-// block, like a butterfly, without leaving the VALU.  Steps >= 16 cross DPP
-// rows and go through ds_bpermute.
+// block, like a butterfly, without leaving the VALU.  Larger steps cross DPP
+// rows through ds_bpermute.
 template <int M>
 OGJK_DEV int xchg_i32(int v) {
   if constexpr (M == 1) return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);
@@ -91,6 +92,47 @@ OGJK_DEV double xchg(double v) {
   return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
+// The two values a symmetric reduction step m combines, in either order:
+// (own, partner).  Steps 16 and 32 use gfx950's v_permlane16_swap /
+// v_permlane32_swap with both operands = v: the two results hold v's even and
+// odd rows (lower and upper halves), i.e. each lane's own value and its
+// xor-16 (xor-32) partner's, without an LDS round trip.
+template <int M>
+OGJK_DEV void xpair_i32(int v, int& a, int& b) {
+  if constexpr (M == 16) {
+    const auto r = __builtin_amdgcn_permlane16_swap((unsigned)v, (unsigned)v, false, false);
+    a = (int)r[0];
+    b = (int)r[1];
+  } else if constexpr (M == 32) {
+    const auto r = __builtin_amdgcn_permlane32_swap((unsigned)v, (unsigned)v, false, false);
+    a = (int)r[0];
+    b = (int)r[1];
+  } else {
+    a = v;
+    b = xchg_i32<M>(v);
+  }
+}
+template <int M>
+OGJK_DEV void xpair(float v, float& a, float& b) {
+  int x, y;
+  xpair_i32<M>(__builtin_bit_cast(int, v), x, y);
+  a = __builtin_bit_cast(float, x);
+  b = __builtin_bit_cast(float, y);
+}
+template <int M>
+OGJK_DEV void xpair(int v, int& a, int& b) {
+  xpair_i32<M>(v, a, b);
+}
+template <int M>
+OGJK_DEV void xpair(double v, double& a, double& b) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  int lo0, lo1, hi0, hi1;
+  xpair_i32<M>((int)(uint32_t)u, lo0, lo1);
+  xpair_i32<M>((int)(uint32_t)(u >> 32), hi0, hi1);
+  a = __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi0 << 32) | (uint32_t)lo0);
+  b = __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi1 << 32) | (uint32_t)lo1);
+}
+
 template <int LPP>
 OGJK_DEV double shfl_d(double v, int src) {
   return __shfl(v, src, LPP);
@@ -107,7 +149,9 @@ OGJK_DEV float shfl_f(float v, int src) {
 template <int LPP, int M = 1>
 OGJK_DEV float group_max_f(float v) {
   if constexpr (M < LPP) {
-    v = fmaxf(v, xchg<M>(v));
+    float a, b;
+    xpair<M>(v, a, b);
+    v = fmaxf(a, b);
     return group_max_f<LPP, 2 * M>(v);
   } else {
     return v;
@@ -119,12 +163,13 @@ OGJK_DEV float group_max_f(float v) {
 template <int LPP, int M = 1>
 OGJK_DEV void group_argmax_d(double& v, int& i) {
   if constexpr (M < LPP) {
-    const double ov = xchg<M>(v);
-    const int oi = xchg<M>(i);
-    if (ov > v || (ov == v && oi < i)) {
-      v = ov;
-      i = oi;
-    }
+    double va, vb;
+    int ia, ib;
+    xpair<M>(v, va, vb);
+    xpair<M>(i, ia, ib);
+    const bool tb = vb > va || (vb == va && ib < ia);  // symmetric in (a, b)
+    v = tb ? vb : va;
+    i = tb ? ib : ia;
     group_argmax_d<LPP, 2 * M>(v, i);
   }
 }
@@ -133,12 +178,13 @@ OGJK_DEV void group_argmax_d(double& v, int& i) {
 template <int LPP, int M = 1>
 OGJK_DEV void group_argmin_d(double& v, int& i) {
   if constexpr (M < LPP) {
-    const double ov = xchg<M>(v);
-    const int oi = xchg<M>(i);
-    if (ov < v || (ov == v && oi < i)) {
-      v = ov;
-      i = oi;
-    }
+    double va, vb;
+    int ia, ib;
+    xpair<M>(v, va, vb);
+    xpair<M>(i, ia, ib);
+    const bool tb = vb < va || (vb == va && ib < ia);  // symmetric in (a, b)
+    v = tb ? vb : va;
+    i = tb ? ib : ia;
     group_argmin_d<LPP, 2 * M>(v, i);
   }
 }

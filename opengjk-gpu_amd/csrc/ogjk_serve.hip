@@ -267,7 +267,7 @@ struct ogjk_server_s {
           if (q == hipSuccess) {
             if (r.simplices) std::memcpy(r.simplices, s.h_simp + o, sizeof(gkSimplex) * n);
             if (r.distances) std::memcpy(r.distances, s.h_dist + o, 4 * n);
-            if (r.normals) std::memcpy(r.normals, s.h_nrm + o, 12 * n);
+            if (r.normals) std::memcpy(r.normals, s.h_nrm + 3 * o, 12 * n);
             if (r.f64) std::memcpy(r.f64, s.h_f64 + OGJK_F64_RECORD * o, 8 * OGJK_F64_RECORD * n);
           }
         }

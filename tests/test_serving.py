@@ -91,6 +91,12 @@ def test_alone_vs_mix(torch_cuda, scene):
         srv.close()
     mixed, _, batch = res[probe]
     shared = sum(1 for k, r in enumerate(res) if r[2] == batch and k != probe)
+    # at least one request of the mix sat behind others in its launch, so the
+    # per-request offsets into the batch buffers are exercised
+    firsts = {}
+    for k, r in enumerate(res):
+        firsts.setdefault(r[2], k)
+    assert any(firsts[r[2]] != k for k, r in enumerate(res))
     print(f"\n[mix] launches {info['launches']} for {len(reqs) + 1} requests; probe shared its "
           f"batch with {shared} others", flush=True)
     assert info["launches"] < len(reqs) + 1
@@ -99,7 +105,8 @@ def test_alone_vs_mix(torch_cuda, scene):
     assert np.array_equal(alone["simplices"].view(np.uint8), mixed["simplices"].view(np.uint8))
     assert np.array_equal(alone["distances"], mixed["distances"])
     assert np.array_equal(alone["normals"], mixed["normals"])
-    # and every request of the mix against the oracle, on a sample of pairs
+    # and every request of the mix against the oracle, on a sample of pairs,
+    # including the float outputs each request receives
     for k in (0, 7, probe, 33):
         pairs, poses = reqs[k]
         m = min(len(pairs), 400)
@@ -107,3 +114,5 @@ def test_alone_vs_mix(torch_cuda, scene):
         out = res[k][0]
         r = compare_f64(out["f64"][:m], out["simplices"][:m], ref, oracle.GJK_EPA)
         assert r["exact"] == r["n"], (k, r["inexact"])
+        assert np.array_equal(out["distances"][:m], ref.distances.astype(np.float32))
+        assert np.array_equal(out["normals"][:m], ref.normals.astype(np.float32))
