@@ -61,6 +61,11 @@ typedef struct ogjk_clusters {
                           half extents x, y, z, unused */
   int64_t n_slots;     /* capacity of x, y, z, orig: >= ogjk_cluster_slots() */
   int32_t min_verts;   /* smaller bodies are swept without culling */
+  float* super_bounds; /* 8 * n_supers: bounds of each run of 32 clusters of the
+                          bodies of more than 4,096 vertices, as bounds */
+  int64_t* body_super; /* n_bodies: first supercluster of each body, -1: none */
+  int64_t n_supers;    /* capacity of super_bounds: >= ogjk_cluster_supers(n_slots) */
+  int64_t n_supers_used; /* set by the build: superclusters written */
 } ogjk_clusters;
 
 typedef struct ogjk_scene {
@@ -198,6 +203,8 @@ ogjk_status ogjk_server_destroy(ogjk_server server);
 /* Slots a cluster build needs for a scene of n_verts vertices in n_bodies
  * bodies (padding to 32 per body included). */
 int64_t ogjk_cluster_slots(int64_t n_verts, int32_t n_bodies);
+/* Superclusters (runs of 32 clusters) a build over n_slots slots can need. */
+int64_t ogjk_cluster_supers(int64_t n_slots);
 
 /* Builds the clusters of every body of a packed scene (scene->x/y/z,
  * body_off, body_cnt) into caller-allocated arrays.  Synchronous on stream;

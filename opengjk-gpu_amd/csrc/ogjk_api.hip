@@ -275,8 +275,10 @@ static ogjk_status scene_args(const ogjk_scene* scene, const int32_t* d_pairs, i
   *sa = ogjk::SceneArgs{scene->x, scene->y, scene->z, scene->body_off, scene->body_cnt,
                         scene->body_r1, scene->n_bodies, d_pairs};
   if (const ogjk_clusters* cl = scene->clusters) {
-    if (!cl->x || !cl->y || !cl->z || !cl->orig || !cl->body_slot || !cl->bounds)
+    if (!cl->x || !cl->y || !cl->z || !cl->orig || !cl->body_slot || !cl->bounds ||
+        !cl->super_bounds || !cl->body_super)
       return fail(OGJK_ERR_INVALID, "incomplete clusters");
+    sa->csbounds = cl->n_supers_used > 0 ? cl->super_bounds : nullptr;
     sa->cx = cl->x;
     sa->cy = cl->y;
     sa->cz = cl->z;
@@ -390,20 +392,24 @@ int64_t ogjk_cluster_slots(int64_t n_verts, int32_t n_bodies) {
   return ogjk::cluster_slots(n_verts, n_bodies);
 }
 
+int64_t ogjk_cluster_supers(int64_t n_slots) { return ogjk::cluster_supers(n_slots); }
+
 ogjk_status ogjk_build_clusters(const ogjk_scene* scene, ogjk_clusters* cl, void* stream) {
   if (!scene || !cl) return fail(OGJK_ERR_INVALID, "NULL scene or clusters");
   if (scene->n_bodies < 0) return fail(OGJK_ERR_INVALID, "negative body count");
   if (scene->n_bodies == 0) return OGJK_OK;
   if (!scene->x || !scene->y || !scene->z || !scene->body_off || !scene->body_cnt)
     return fail(OGJK_ERR_INVALID, "incomplete scene");
-  if (!cl->x || !cl->y || !cl->z || !cl->orig || !cl->body_slot || !cl->bounds)
+  if (!cl->x || !cl->y || !cl->z || !cl->orig || !cl->body_slot || !cl->bounds ||
+      !cl->super_bounds || !cl->body_super)
     return fail(OGJK_ERR_INVALID, "incomplete clusters");
   const hipError_t e = ogjk::build_clusters(
       scene->x, scene->y, scene->z, scene->body_off, scene->body_cnt, scene->n_bodies, cl->x,
-      cl->y, cl->z, cl->orig, cl->body_slot, cl->bounds, cl->n_slots, (hipStream_t)stream);
+      cl->y, cl->z, cl->orig, cl->body_slot, cl->bounds, cl->n_slots, cl->super_bounds,
+      cl->body_super, cl->n_supers, &cl->n_supers_used, (hipStream_t)stream);
   if (e == hipErrorInvalidValue)
-    return fail(OGJK_ERR_INVALID, "clusters need more than %lld slots, or a body exceeds 2^27 "
-                "vertices", (long long)cl->n_slots);
+    return fail(OGJK_ERR_INVALID, "clusters need more than %lld slots or %lld superclusters, or "
+                "a body exceeds 2^27 vertices", (long long)cl->n_slots, (long long)cl->n_supers);
   if (e == hipErrorOutOfMemory) return fail(OGJK_ERR_NOMEM, "cluster build scratch");
   OGJK_HIP(e, "cluster build");
   return OGJK_OK;

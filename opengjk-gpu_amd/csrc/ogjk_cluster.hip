@@ -188,7 +188,86 @@ __global__ void __launch_bounds__(256) k_cluster_bounds(const float* __restrict_
   }
 }
 
+OGJK_DEV double block_max_d(double v, double* red) {
+  for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+// One block per supercluster (first slot, slot count <= kSuper*kCluster):
+// sphere and box bounds of its vertices, as k_cluster_bounds does for one
+// cluster.
+__global__ void __launch_bounds__(256) k_super_bounds(const float* __restrict__ cx,
+                                                      const float* __restrict__ cy,
+                                                      const float* __restrict__ cz,
+                                                      const int32_t* __restrict__ orig,
+                                                      const int64_t* __restrict__ first,
+                                                      const int32_t* __restrict__ count,
+                                                      ClusterBound* __restrict__ sbounds) {
+  __shared__ float red[4];
+  __shared__ double redd[4];
+  const int64_t s0 = first[blockIdx.x];
+  const int n = count[blockIdx.x];
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int i = threadIdx.x; i < n; i += 256) {
+    if (orig[s0 + i] < 0) continue;
+    const float p[3] = {cx[s0 + i], cy[s0 + i], cz[s0 + i]};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      lo[k] = fminf(lo[k], p[k]);
+      hi[k] = fmaxf(hi[k], p[k]);
+    }
+  }
+  float ctr[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    lo[k] = block_min(lo[k], red);
+    hi[k] = block_max(hi[k], red);
+    ctr[k] = 0.5f * lo[k] + 0.5f * hi[k];
+  }
+  double r = 0.0, e[3] = {0.0, 0.0, 0.0};
+  for (int i = threadIdx.x; i < n; i += 256) {
+    if (orig[s0 + i] < 0) continue;
+    const double ex = fabs((double)cx[s0 + i] - (double)ctr[0]);
+    const double ey = fabs((double)cy[s0 + i] - (double)ctr[1]);
+    const double ez = fabs((double)cz[s0 + i] - (double)ctr[2]);
+    r = fmax(r, sqrt(ex * ex + ey * ey + ez * ez));
+    e[0] = fmax(e[0], ex);
+    e[1] = fmax(e[1], ey);
+    e[2] = fmax(e[2], ez);
+  }
+  r = block_max_d(r, redd);
+#pragma unroll
+  for (int k = 0; k < 3; k++) e[k] = block_max_d(e[k], redd);
+  if (threadIdx.x == 0) {
+    ClusterBound cb;
+    cb.c = make_float4(ctr[0], ctr[1], ctr[2], nextafterf((float)r, INFINITY));
+    cb.h = make_float4(nextafterf((float)e[0], INFINITY), nextafterf((float)e[1], INFINITY),
+                       nextafterf((float)e[2], INFINITY), 0.0f);
+    sbounds[blockIdx.x] = cb;
+  }
+}
+
+// Each body's first supercluster index goes into the spare word of its first
+// cluster's bound (read by the culled support query).
+__global__ void __launch_bounds__(256) k_mark_super(const int32_t* __restrict__ cnt,
+                                                    const int64_t* __restrict__ slot,
+                                                    const int64_t* __restrict__ super,
+                                                    int n_bodies, ClusterBound* __restrict__ b) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_bodies || cnt[i] <= 0) return;
+  b[slot[i] / kCluster].h.w = __builtin_bit_cast(float, (int)super[i]);
+}
+
 }  // namespace
+
+// Superclusters a scene of n_slots cluster slots can need: only bodies of
+// more than kSuperMinClusters clusters have them (ceil(nc / kSuper) each).
+int64_t cluster_supers(int64_t n_slots) {
+  return 2 * (n_slots / (kSuper * kCluster)) + 1;
+}
 
 int64_t cluster_slots(int64_t n_verts, int32_t n_bodies) {
   return n_verts + (int64_t)(kCluster - 1) * (n_bodies > 0 ? n_bodies : 0);
@@ -200,7 +279,8 @@ int64_t cluster_slots(int64_t n_verts, int32_t n_bodies) {
 hipError_t build_clusters(const float* x, const float* y, const float* z, const int64_t* off,
                           const int32_t* cnt, int32_t n_bodies, float* cx, float* cy, float* cz,
                           int32_t* orig, int64_t* body_slot, float* bounds, int64_t n_slots,
-                          hipStream_t st) {
+                          float* sbounds, int64_t* body_super, int64_t n_supers,
+                          int64_t* n_supers_used, hipStream_t st) {
   constexpr int64_t kBatch = int64_t(1) << 27;
   if (n_bodies <= 0) return hipSuccess;
   std::vector<int32_t> hcnt(n_bodies);
@@ -218,6 +298,25 @@ hipError_t build_clusters(const float* x, const float* y, const float* z, const 
   if (total > n_slots || maxn > kBatch) return hipErrorInvalidValue;
   e = hipMemcpyAsync(body_slot, hslot.data(), sizeof(int64_t) * n_bodies, hipMemcpyHostToDevice,
                      st);
+  if (e != hipSuccess) return e;
+  // superclusters of the bodies with more than kSuper clusters
+  std::vector<int64_t> hsuper(n_bodies, -1), sfirst;
+  std::vector<int32_t> scount;
+  for (int b = 0; b < n_bodies; b++) {
+    const int64_t n = hcnt[b] > 0 ? hcnt[b] : 0;
+    const int64_t nc = (n + kCluster - 1) / kCluster;
+    if (nc <= kSuperMinClusters) continue;
+    hsuper[b] = (int64_t)sfirst.size();
+    for (int64_t c0 = 0; c0 < nc; c0 += kSuper) {
+      sfirst.push_back(hslot[b] + c0 * kCluster);
+      scount.push_back((int32_t)(std::min<int64_t>(kSuper, nc - c0) * kCluster));
+    }
+  }
+  if ((int64_t)sfirst.size() > n_supers || (int64_t)sfirst.size() > 0x7fffffff)
+    return hipErrorInvalidValue;
+  *n_supers_used = (int64_t)sfirst.size();
+  e = hipMemcpyAsync(body_super, hsuper.data(), sizeof(int64_t) * n_bodies,
+                     hipMemcpyHostToDevice, st);
   if (e != hipSuccess) return e;
 
   // batches of consecutive bodies
@@ -252,12 +351,12 @@ hipError_t build_clusters(const float* x, const float* y, const float* z, const 
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
   auto cleanup = [&]() {
-    hipFree(keys);
-    hipFree(keys2);
-    hipFree(vals);
-    hipFree(vals2);
-    hipFree(kbeg);
-    hipFree(tmp);
+    (void)hipFree(keys);
+    (void)hipFree(keys2);
+    (void)hipFree(vals);
+    (void)hipFree(vals2);
+    (void)hipFree(kbeg);
+    (void)hipFree(tmp);
   };
   if ((e = hipMalloc(&keys, 4 * maxv)) != hipSuccess || (e = hipMalloc(&keys2, 4 * maxv)) ||
       (e = hipMalloc(&vals, 4 * maxv)) || (e = hipMalloc(&vals2, 4 * maxv)) ||
@@ -287,7 +386,7 @@ hipError_t build_clusters(const float* x, const float* y, const float* z, const 
       return e;
     }
     if (need > tmp_bytes) {
-      hipFree(tmp);
+      (void)hipFree(tmp);
       tmp = nullptr;
       if ((e = hipMalloc(&tmp, need)) != hipSuccess) {
         cleanup();
@@ -315,8 +414,35 @@ hipError_t build_clusters(const float* x, const float* y, const float* z, const 
     hipLaunchKernelGGL(k_cluster_bounds, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st,
                        cx, cy, cz, orig, n_clusters, (ClusterBound*)bounds);
   }
-  e = hipGetLastError();
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  hipLaunchKernelGGL(k_mark_super, dim3((unsigned)((n_bodies + 255) / 256)), dim3(256), 0, st, cnt,
+                     body_slot, body_super, n_bodies, (ClusterBound*)bounds);
+  if ((e = hipGetLastError()) != hipSuccess) {
+    cleanup();
+    return e;
+  }
+  if (!sfirst.empty()) {
+    int64_t* d_first = nullptr;
+    int32_t* d_count = nullptr;
+    const size_t ns = sfirst.size();
+    if ((e = hipMalloc(&d_first, 8 * ns)) == hipSuccess &&
+        (e = hipMalloc(&d_count, 4 * ns)) == hipSuccess &&
+        (e = hipMemcpyAsync(d_first, sfirst.data(), 8 * ns, hipMemcpyHostToDevice, st)) ==
+            hipSuccess &&
+        (e = hipMemcpyAsync(d_count, scount.data(), 4 * ns, hipMemcpyHostToDevice, st)) ==
+            hipSuccess) {
+      hipLaunchKernelGGL(k_super_bounds, dim3((unsigned)ns), dim3(256), 0, st, cx, cy, cz, orig,
+                         d_first, d_count, (ClusterBound*)sbounds);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+    }
+    (void)hipFree(d_first);
+    (void)hipFree(d_count);
+    if (e != hipSuccess) {
+      cleanup();
+      return e;
+    }
+  }
+  e = hipStreamSynchronize(st);
   cleanup();
   return e;
 }

@@ -208,7 +208,16 @@ OGJK_DEV Sup support(const Body<float, STRIDE>& A, const Query& q, int lane) {
 // lies more than 2^-16 r1 below the maximum: it can neither be the FP64
 // argmax nor tie with it, and the selection is the full sweep's.  Indices
 // reported are the body-local input indices (the copy stores them).
+//
+// Bodies of more than kSuper clusters also carry superclusters (kSuper
+// consecutive clusters, bounded the same way from their vertices): the query
+// bounds the superclusters first and only opens those whose bound reaches
+// the threshold, so a 100,000-vertex body reads ~100 supercluster bounds per
+// query instead of ~3,000 cluster bounds.  A supercluster's bound covers
+// every vertex of its clusters, so skipping it is exact by the same argument.
 constexpr int kCluster = 32;
+constexpr int kSuper = 32;
+constexpr int kSuperMinClusters = 4 * kSuper;  // superclusters above 4,096 vertices
 
 // Bounds of one cluster: centre c and radius r (c.w) of a sphere, and half
 // extents h of the box around c, all rounded outwards.
@@ -221,14 +230,24 @@ struct ClusterView {
   const float* x;  // cluster-ordered vertices of every body
   const float* y;
   const float* z;
-  const int32_t* orig;         // body-local input index of each slot
-  const ClusterBound* bounds;  // per cluster
+  const int32_t* orig;          // body-local input index of each slot
+  const ClusterBound* bounds;   // per cluster
+  const ClusterBound* sbounds;  // per supercluster
 };
 
 struct CBody : Body<float, 1> {
   ClusterView cv;
   int64_t slot;  // first slot of the body (a multiple of kCluster); < 0: no culling
 };
+
+// A clustered body of a scene that has superclusters (two-level queries).
+struct CBody2 : CBody {};
+
+// The body's first supercluster (< 0: none) is kept in the spare word of its
+// first cluster's bound, so a body carries no extra field through GJK.
+OGJK_DEV int first_super(const ClusterBound* body_bounds) {
+  return __builtin_bit_cast(int, body_bounds[0].h.w);
+}
 
 // c.d + min(r |d|, h.|d|) >= max over the cluster of x.d
 OGJK_DEV float cluster_ub(const ClusterBound& b, float dx, float dy, float dz, float dn) {
@@ -237,14 +256,15 @@ OGJK_DEV float cluster_ub(const ClusterBound& b, float dx, float dy, float dz, f
   return cd + fminf(b.c.w * dn, box);
 }
 
-// Calls f(k) for every body-local slot k of the clusters whose bound reaches
-// L, each slot on exactly one lane, in the same order on every call.
+// Calls f(k) for every body-local slot k of the clusters c0 <= c < c0 + ncl
+// whose bound reaches L (cb: the body's cluster bounds), each slot on exactly
+// one lane, in the same order on every call.
 template <int LPP, class F>
-OGJK_DEV void visit_clusters(const CBody& A, const ClusterBound* cb, int nc, float dx, float dy,
-                             float dz, float dn, float L, int lane, F&& f) {
-  for (int base = 0; base < nc; base += LPP) {
+OGJK_DEV void visit_range(const CBody& A, const ClusterBound* cb, int c0, int ncl, float dx,
+                          float dy, float dz, float dn, float L, int lane, F&& f) {
+  for (int base = 0; base < ncl; base += LPP) {
     const int c = base + lane;
-    const bool live = c < nc && !(cluster_ub(cb[c], dx, dy, dz, dn) < L);
+    const bool live = c < ncl && !(cluster_ub(cb[c0 + c], dx, dy, dz, dn) < L);
     uint64_t m = group_ballot<LPP>(live);
     while (m != 0) {
       if constexpr (LPP >= kCluster) {
@@ -259,7 +279,7 @@ OGJK_DEV void visit_clusters(const CBody& A, const ClusterBound* cb, int nc, flo
           }
         }
         if (cid >= 0) {
-          const int k = (base + cid) * kCluster + (lane & (kCluster - 1));
+          const int k = (c0 + base + cid) * kCluster + (lane & (kCluster - 1));
           if (k < A.n) f(k);
         }
       } else {
@@ -267,7 +287,7 @@ OGJK_DEV void visit_clusters(const CBody& A, const ClusterBound* cb, int nc, flo
         m &= m - 1;
 #pragma unroll
         for (int t = lane; t < kCluster; t += LPP) {
-          const int k = (base + cid) * kCluster + t;
+          const int k = (c0 + base + cid) * kCluster + t;
           if (k < A.n) f(k);
         }
       }
@@ -275,8 +295,52 @@ OGJK_DEV void visit_clusters(const CBody& A, const ClusterBound* cb, int nc, flo
   }
 }
 
+// All surviving clusters of the body: through its superclusters when it has
+// them (sb non-null), in supercluster then cluster order; without, the body
+// is a single unit of nc clusters.  One loop for both keeps one copy of the
+// cluster visit per call site.
+template <int LPP, bool TWO, class F>
+OGJK_DEV void visit_clusters(const CBody& A, const ClusterBound* cb, int nc,
+                             const ClusterBound* sb, float dx, float dy, float dz, float dn,
+                             float L, int lane, F&& f) {
+  if constexpr (!TWO) {
+    visit_range<LPP>(A, cb, 0, nc, dx, dy, dz, dn, L, lane, f);
+    return;
+  }
+  const int nunits = sb != nullptr ? (nc + kSuper - 1) / kSuper : 1;
+  for (int base = 0; base < nunits; base += LPP) {
+    const int u = base + lane;
+    const bool live = u < nunits && (sb == nullptr || !(cluster_ub(sb[u], dx, dy, dz, dn) < L));
+    uint64_t m = group_ballot<LPP>(live);
+    while (m != 0) {
+      const int u0 = base + (int)__builtin_ctzll(m);
+      m &= m - 1;
+      const int c0 = sb != nullptr ? u0 * kSuper : 0;
+      const int ncl = sb != nullptr ? min(kSuper, nc - c0) : nc;
+      visit_range<LPP>(A, cb, c0, ncl, dx, dy, dz, dn, L, lane, f);
+    }
+  }
+}
+
+// Index (within [0, n)) of the entry of b with the largest bound.
 template <int LPP>
-OGJK_DEV Sup support(const CBody& A, const Query& q, int lane) {
+OGJK_DEV int best_bound(const ClusterBound* b, int n, float dx, float dy, float dz, float dn,
+                        int lane) {
+  float bub = -INFINITY;
+  int bi = 0;
+  for (int c = lane; c < n; c += LPP) {
+    const float ub = cluster_ub(b[c], dx, dy, dz, dn);
+    if (ub > bub) {
+      bub = ub;
+      bi = c;
+    }
+  }
+  const float gub = group_max_f<LPP>(bub);
+  return shfl_i<LPP>(bi, (int)__builtin_ctzll(group_ballot<LPP>(bub == gub) | (1ull << 63)));
+}
+
+template <int LPP, bool TWO>
+OGJK_DEV Sup support_clustered(const CBody& A, const Query& q, int lane) {
   if (A.slot < 0) return support<LPP>(static_cast<const Body<float, 1>&>(A), q, lane);
   Sweep32 sw = sweep32_begin(q.dir);
   const float dx = sw.dx, dy = sw.dy, dz = sw.dz;
@@ -287,18 +351,20 @@ OGJK_DEV Sup support(const CBody& A, const Query& q, int lane) {
   const float* sy = A.cv.y + A.slot;
   const float* sz = A.cv.z + A.slot;
 
-  // 1-2: the cluster with the largest bound gives a lower bound of the max
-  float bub = -INFINITY;
-  int bc = 0;
-  for (int c = lane; c < nc; c += LPP) {
-    const float ub = cluster_ub(cb[c], dx, dy, dz, dn);
-    if (ub > bub) {
-      bub = ub;
-      bc = c;
-    }
+  const ClusterBound* sb = nullptr;
+  if constexpr (TWO) {
+    const int sup = first_super(cb);
+    if (sup >= 0) sb = A.cv.sbounds + sup;
   }
-  const float gub = group_max_f<LPP>(bub);
-  bc = shfl_i<LPP>(bc, (int)__builtin_ctzll(group_ballot<LPP>(bub == gub) | (1ull << 63)));
+
+  // 1-2: the cluster with the largest bound (inside the supercluster with the
+  // largest bound, when there are superclusters) gives a lower bound of the max
+  int c0 = 0, ncl = nc;
+  if (TWO && sb != nullptr) {
+    c0 = best_bound<LPP>(sb, (nc + kSuper - 1) / kSuper, dx, dy, dz, dn, lane) * kSuper;
+    ncl = min(kSuper, nc - c0);
+  }
+  const int bc = c0 + best_bound<LPP>(cb + c0, ncl, dx, dy, dz, dn, lane);
   float m0 = -INFINITY;
 #pragma unroll
   for (int t = lane; t < kCluster; t += LPP) {
@@ -308,7 +374,7 @@ OGJK_DEV Sup support(const CBody& A, const Query& q, int lane) {
   const float L = group_max_f<LPP>(m0) - 0x1.0p-15f * A.r1;
 
   // 3: sweep the surviving clusters
-  visit_clusters<LPP>(A, cb, nc, dx, dy, dz, dn, L, lane, [&](int k) {
+  visit_clusters<LPP, TWO>(A, cb, nc, sb, dx, dy, dz, dn, L, lane, [&](int k) {
     const float px = sx[k], py = sy[k], pz = sz[k];
     const float d = fmaf(px, dx, fmaf(py, dy, pz * dz));
     sw.second = fmaxf(sw.second, fminf(sw.best, d));
@@ -334,7 +400,7 @@ OGJK_DEV Sup support(const CBody& A, const Query& q, int lane) {
       c.i = orig[sw.bi];
     }
   } else {
-    visit_clusters<LPP>(A, cb, nc, dx, dy, dz, dn, L, lane, [&](int k) {
+    visit_clusters<LPP, TWO>(A, cb, nc, sb, dx, dy, dz, dn, L, lane, [&](int k) {
       const float px = sx[k], py = sy[k], pz = sz[k];
       const float d = fmaf(px, dx, fmaf(py, dy, pz * dz));
       if (d >= T) {
@@ -351,6 +417,15 @@ OGJK_DEV Sup support(const CBody& A, const Query& q, int lane) {
   }
   if (!(c.v == c.v)) c.v = -INFINITY;
   return finish1<LPP, false>(c, q);
+}
+
+template <int LPP>
+OGJK_DEV Sup support(const CBody& A, const Query& q, int lane) {
+  return support_clustered<LPP, false>(A, q, lane);
+}
+template <int LPP>
+OGJK_DEV Sup support(const CBody2& A, const Query& q, int lane) {
+  return support_clustered<LPP, true>(A, q, lane);
 }
 
 // ------------------------------------------------------------ posed bodies
@@ -451,6 +526,9 @@ OGJK_DEV Body<T, STRIDE> pick(bool c, const Body<T, STRIDE>& a, const Body<T, ST
 OGJK_DEV CBody pick(bool c, const CBody& a, const CBody& b) {
   return CBody{pick(c, static_cast<const Body<float, 1>&>(a), static_cast<const Body<float, 1>&>(b)),
                a.cv, c ? a.slot : b.slot};
+}
+OGJK_DEV CBody2 pick(bool c, const CBody2& a, const CBody2& b) {
+  return CBody2{pick(c, static_cast<const CBody&>(a), static_cast<const CBody&>(b))};
 }
 OGJK_DEV PBody pick(bool c, const PBody& a, const PBody& b) {
   return PBody{pick(c, static_cast<const Body<float, 1>&>(a), static_cast<const Body<float, 1>&>(b)),

@@ -27,6 +27,7 @@ struct SceneArgs {       // SoA float scene (or double for the plain path)
   const int32_t* corig = nullptr;
   const int64_t* cslot = nullptr;
   const float* cbounds = nullptr;
+  const float* csbounds = nullptr;  // supercluster bounds; null: the scene has none
   int32_t cluster_min = 0x7fffffff;
 };
 
@@ -86,10 +87,12 @@ hipError_t launch_posed(int mode, const SceneArgs& s, const float* poses, const 
                         int n, void* ws, hipStream_t st);
 // Vertex clusters (see ogjk_cluster.hip): slots needed, and the build.
 int64_t cluster_slots(int64_t n_verts, int32_t n_bodies);
+int64_t cluster_supers(int64_t n_slots);
 hipError_t build_clusters(const float* x, const float* y, const float* z, const int64_t* off,
                           const int32_t* cnt, int32_t n_bodies, float* cx, float* cy, float* cz,
                           int32_t* orig, int64_t* body_slot, float* bounds, int64_t n_slots,
-                          hipStream_t st);
+                          float* sbounds, int64_t* body_super, int64_t n_supers,
+                          int64_t* n_supers_used, hipStream_t st);
 hipError_t launch_pack_f64(const float* aos, int64_t n_verts, double* x, double* y, double* z,
                            hipStream_t st);
 hipError_t launch_pack(const float* aos, const int64_t* off, const int32_t* cnt, int n_bodies,

@@ -23,9 +23,9 @@ namespace ogjk {
 
 // ----------------------------------------------------------------- sources
 
-// CL: bodies carry their vertex clusters (float scenes built with clusters);
-// without, the kernels contain no culled sweep at all.
-template <class T, bool CL = false>
+// CL: 0 no clusters (the kernels contain no culled sweep at all), 1 clusters,
+// 2 clusters and superclusters (float scenes only).
+template <class T, int CL = 0>
 struct SceneSrc {
   const T* x;
   const T* y;
@@ -38,10 +38,11 @@ struct SceneSrc {
   ClusterView cv;        // float scene: vertex clusters, or null bounds
   const int64_t* cslot;  // first cluster slot of each body
   int32_t cluster_min;   // bodies with at least this many vertices are culled
-  using BodyT = std::conditional_t<CL, CBody, Body<T, 1>>;
+  using BodyT =
+      std::conditional_t<CL == 2, CBody2, std::conditional_t<CL == 1, CBody, Body<T, 1>>>;
   static constexpr int kGjkWaves = CL ? OGJK_GJK_WAVES_WIDE : OGJK_GJK_WAVES;
   // GJK stages small pairs in LDS (float scenes without clusters)
-  static constexpr bool kStage = std::is_same_v<T, float> && !CL;
+  static constexpr bool kStage = std::is_same_v<T, float> && CL == 0;
 
   // max(nA, nB) of pair i, or 0 for an invalid pair (it goes to bucket 0)
   OGJK_DEV int pair_size(int i) const {
@@ -64,9 +65,11 @@ struct SceneSrc {
     const int64_t oa = off[a], ob = off[b];
     const int na = cnt[a], nb_ = cnt[b];
     const float ra = r1 ? r1[a] : 0.0f, rb = r1 ? r1[b] : 0.0f;
-    if constexpr (CL) {
-      A = CBody{{x + oa, y + oa, z + oa, na, ra}, cv, na >= cluster_min ? cslot[a] : int64_t(-1)};
-      B = CBody{{x + ob, y + ob, z + ob, nb_, rb}, cv, nb_ >= cluster_min ? cslot[b] : int64_t(-1)};
+    if constexpr (CL != 0) {
+      A = BodyT{
+          CBody{{x + oa, y + oa, z + oa, na, ra}, cv, na >= cluster_min ? cslot[a] : int64_t(-1)}};
+      B = BodyT{CBody{
+          {x + ob, y + ob, z + ob, nb_, rb}, cv, nb_ >= cluster_min ? cslot[b] : int64_t(-1)}};
     } else {
       A = BodyT{x + oa, y + oa, z + oa, na, ra};
       B = BodyT{x + ob, y + ob, z + ob, nb_, rb};
@@ -792,11 +795,16 @@ static hipError_t launch_any(int mode, const Src& src, const OutArgs& o, int n, 
 
 hipError_t launch_scene(int mode, const SceneArgs& s, const OutArgs& o, int n, void* ws,
                         hipStream_t st) {
-  const ClusterView cv{s.cx, s.cy, s.cz, s.corig, (const ClusterBound*)s.cbounds};
+  const ClusterView cv{s.cx, s.cy, s.cz, s.corig, (const ClusterBound*)s.cbounds,
+                       (const ClusterBound*)s.csbounds};
+  if (s.cbounds && s.csbounds) {
+    const SceneSrc<float, 2> src{(const float*)s.x, (const float*)s.y, (const float*)s.z, s.off,
+                                 s.cnt, s.r1, s.n_bodies, s.pairs, cv, s.cslot, s.cluster_min};
+    return launch_any(mode, src, o, n, ws, st);
+  }
   if (s.cbounds) {
-    const SceneSrc<float, true> src{(const float*)s.x, (const float*)s.y, (const float*)s.z,
-                                    s.off, s.cnt, s.r1, s.n_bodies, s.pairs, cv, s.cslot,
-                                    s.cluster_min};
+    const SceneSrc<float, 1> src{(const float*)s.x, (const float*)s.y, (const float*)s.z, s.off,
+                                 s.cnt, s.r1, s.n_bodies, s.pairs, cv, s.cslot, s.cluster_min};
     return launch_any(mode, src, o, n, ws, st);
   }
   const SceneSrc<float> src{(const float*)s.x, (const float*)s.y, (const float*)s.z, s.off,

@@ -38,7 +38,7 @@ C_ABI_SYMBOLS = [
     "ogjk_set_config", "ogjk_set_buckets", "ogjk_workspace_bytes", "ogjk_collide_scene_ws", "ogjk_pack_scene",
     "ogjk_collide_scene", "ogjk_pack_scene_f64", "ogjk_collide_scene_f64",
     "ogjk_collide_polytopes", "ogjk_collide_polytopes_indexed", "ogjk_collide_host",
-    "ogjk_collide_host_indexed", "ogjk_cluster_slots", "ogjk_build_clusters",
+    "ogjk_collide_host_indexed", "ogjk_cluster_slots", "ogjk_cluster_supers", "ogjk_build_clusters",
     "ogjk_graph_capture_scene", "ogjk_graph_launch", "ogjk_graph_destroy", "ogjk_collide_posed",
     "ogjk_server_create", "ogjk_server_submit", "ogjk_server_wait", "ogjk_server_poll",
     "ogjk_server_info", "ogjk_server_destroy",
@@ -52,7 +52,9 @@ class OgjkError(RuntimeError):
 class _Clusters(C.Structure):
     _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("orig", C.c_void_p),
                 ("body_slot", C.c_void_p), ("bounds", C.c_void_p), ("n_slots", C.c_int64),
-                ("min_verts", C.c_int32)]
+                ("min_verts", C.c_int32), ("super_bounds", C.c_void_p),
+                ("body_super", C.c_void_p), ("n_supers", C.c_int64),
+                ("n_supers_used", C.c_int64)]
 
 
 class _Scene(C.Structure):
@@ -94,6 +96,7 @@ def lib():
             "ogjk_pack_scene": (C.c_int, [vp, vp, vp, i32, i64, vp, vp, vp, vp, vp]),
             "ogjk_pack_scene_f64": (C.c_int, [vp, i64, vp, vp, vp, vp]),
             "ogjk_cluster_slots": (i64, [i64, i32]),
+            "ogjk_cluster_supers": (i64, [i64]),
             "ogjk_graph_capture_scene": (C.c_int, [C.c_int, C.POINTER(_Scene), vp, i32, vp, vp, vp,
                                                    vp, C.POINTER(vp)]),
             "ogjk_graph_launch": (C.c_int, [vp, vp]),
@@ -244,9 +247,13 @@ class DeviceScene:
         self.corig = t.empty(ns, dtype=t.int32, device=dev)
         self.cslot = t.empty(max(self.n_bodies, 1), dtype=t.int64, device=dev)
         self.cbounds = t.empty((ns // 32 + 1) * 8, dtype=t.float32, device=dev)
+        nsup = int(lib().ogjk_cluster_supers(ns))
+        self.csbounds = t.empty(max(nsup, 1) * 8, dtype=t.float32, device=dev)
+        self.csuper = t.empty(max(self.n_bodies, 1), dtype=t.int64, device=dev)
         self.clusters = _Clusters(self.cx.data_ptr(), self.cy.data_ptr(), self.cz.data_ptr(),
                                   self.corig.data_ptr(), self.cslot.data_ptr(),
-                                  self.cbounds.data_ptr(), ns, int(min_verts))
+                                  self.cbounds.data_ptr(), ns, int(min_verts),
+                                  self.csbounds.data_ptr(), self.csuper.data_ptr(), nsup, 0)
         _check(lib().ogjk_build_clusters(C.byref(self.scene), C.byref(self.clusters),
                                          self._stream(stream)))
         self.scene.clusters = C.pointer(self.clusters)

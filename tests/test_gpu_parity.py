@@ -44,6 +44,8 @@ WORKLOADS = {
     "c2_ellipsoid_4_4096": dict(family=gen.ELLIPSOID, n_pairs=384, seed=2, box=1.5, nmin=4, nmax=4096),
     "c3_noisy_sphere_2000": dict(family=gen.NOISY_SPHERE, n_pairs=64, seed=3, nverts=2000, box=1.4, sigma=0.01),
 }
+LARGE = {"c3_noisy_sphere_100k": dict(family=gen.NOISY_SPHERE, n_pairs=48, seed=3, nverts=100_000,
+                                      box=1.5, sigma=0.01)}
 
 
 @pytest.mark.parametrize("name", list(WORKLOADS))
@@ -69,9 +71,10 @@ def test_degenerate_parity(torch_cuda, mode, batch):
 # Cluster-culled support sweeps select exactly what the full sweep selects:
 # min_verts 1 culls every body, including the tie-heavy degenerate cases.
 @pytest.mark.parametrize("name,cmin", [("c1_sphere64_box1.3", 1), ("c2_ellipsoid_4_4096", 1),
-                                       ("c2_ellipsoid_4_4096", 512), ("c3_noisy_sphere_2000", 1)])
+                                       ("c2_ellipsoid_4_4096", 512), ("c3_noisy_sphere_2000", 1),
+                                       ("c3_noisy_sphere_100k", 1024)])
 def test_cluster_parity(torch_cuda, name, cmin):
-    b = gen.make(**WORKLOADS[name])
+    b = gen.make(**(WORKLOADS.get(name) or LARGE[name]))
     ref = oracle_run(oracle.GJK_EPA, b)
     rec, simp, d32, n32 = run_scene(torch_cuda, b, oracle.GJK_EPA, cluster_min=cmin)
     check(compare_f64(rec, simp, ref, oracle.GJK_EPA), f"{name} clusters>={cmin}")
@@ -89,10 +92,13 @@ def test_cluster_degenerate_parity(torch_cuda, mode, batch):
 def test_cluster_layout(torch_cuda):
     """Every body's slots hold a permutation of its vertices, padded with -1,
     and every cluster's sphere contains its vertices."""
-    b = gen.make(gen.ELLIPSOID, 200, seed=5, box=1.5, nmin=4, nmax=700)
+    b = gen.make(gen.ELLIPSOID, 200, seed=5, box=1.5, nmin=4, nmax=9000)
     sc = ogjk.DeviceScene(b)
     sc.build_clusters(1)
+    assert sc.clusters.n_supers_used > 0
     torch_cuda.cuda.synchronize()
+    sup = sc.csuper.cpu().numpy()
+    sbounds = sc.csbounds.cpu().numpy().reshape(-1, 8).astype(np.float64)
     slot = sc.cslot.cpu().numpy()
     orig = sc.corig.cpu().numpy()
     cx, cy, cz = (t.cpu().numpy().astype(np.float64) for t in (sc.cx, sc.cy, sc.cz))
@@ -110,6 +116,15 @@ def test_cluster_layout(torch_cuda):
             ctr, r, h = bounds[s0 // 32 + c, :3], bounds[s0 // 32 + c, 3], bounds[s0 // 32 + c, 4:7]
             assert (np.linalg.norm(q - ctr, axis=1) <= r).all()
             assert (np.abs(q - ctr) <= h).all()
+        nc = pad // 32
+        if nc <= 128:
+            assert sup[i] == -1
+        else:  # superclusters: runs of 32 clusters, bounds containing their vertices
+            for k in range(-(-nc // 32)):
+                q = pts[k * 1024:(k + 1) * 1024]
+                ctr, r, h = sbounds[sup[i] + k, :3], sbounds[sup[i] + k, 3], sbounds[sup[i] + k, 4:7]
+                assert (np.linalg.norm(q - ctr, axis=1) <= r).all()
+                assert (np.abs(q - ctr) <= h).all()
 
 
 # Size-bucketed launch tables (device partition, per-bucket lanes, EPA lane
