@@ -180,7 +180,7 @@ def serving(args) -> int:
     k_bytes = float(12 * (cnt[kpn[:, 0]] + cnt[kpn[:, 1]]).sum() + 10_000 * (8 + 56 + 32 + 124))
 
     def run_at(qps, rng_seed):
-        srv = ogjk.Server(scene, max_batch_pairs=65536, slots=4)
+        srv = ogjk.Server(scene, max_batch_pairs=65536, slots=args.slots)
         rng = np.random.default_rng(rng_seed)
         gaps = rng.exponential(1.0 / qps, size=n_req)
         arrive = np.cumsum(gaps)
@@ -294,6 +294,7 @@ def main() -> int:
     ap.add_argument("--workload", default="sphere64", choices=sorted(WORKLOADS) + ["serving"])
     ap.add_argument("--qps", type=int, default=1000, help="serving: request rate")
     ap.add_argument("--requests", type=int, default=2000, help="serving: requests per rate")
+    ap.add_argument("--slots", type=int, default=8, help="serving: launches in flight")
     ap.add_argument("--pairs", type=int, default=0, help="override the pair count")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-FP64 comparison")

@@ -54,22 +54,22 @@ struct Request {
   ogjk_status status = OGJK_OK;
 };
 
+// Per batch of `total` pairs, inputs and results are packed so that each
+// direction takes one copy: in = pairs (8 B) then poses (56 B) per pair; out =
+// simplices (108 B), then distances (4 B), then normals (12 B) per pair.
+constexpr size_t kInBytes = 64, kOutBytes = sizeof(gkSimplex) + 16;
+
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
-  int32_t* d_pairs = nullptr;
-  float* d_poses = nullptr;
-  gkSimplex* d_simp = nullptr;
-  float* d_dist = nullptr;
-  float* d_nrm = nullptr;
+  char* d_in = nullptr;
+  char* d_out = nullptr;
   double* d_f64 = nullptr;
   void* d_ws = nullptr;
-  int32_t* h_pairs = nullptr;
-  float* h_poses = nullptr;
-  gkSimplex* h_simp = nullptr;
-  float* h_dist = nullptr;
-  float* h_nrm = nullptr;
+  char* h_in = nullptr;
+  char* h_out = nullptr;
   double* h_f64 = nullptr;
+  int32_t total = 0;
   std::vector<std::shared_ptr<Request>> reqs;
   std::vector<int32_t> offs;
   bool busy = false;      // taken by the dispatcher
@@ -106,18 +106,12 @@ struct ogjk_server_s {
     if (completer.joinable()) completer.join();
     for (Slot& s : slots) {
       if (s.stream) (void)hipStreamSynchronize(s.stream);
-      (void)hipFree(s.d_pairs);
-      (void)hipFree(s.d_poses);
-      (void)hipFree(s.d_simp);
-      (void)hipFree(s.d_dist);
-      (void)hipFree(s.d_nrm);
+      (void)hipFree(s.d_in);
+      (void)hipFree(s.d_out);
       (void)hipFree(s.d_f64);
       (void)hipFree(s.d_ws);
-      (void)hipHostFree(s.h_pairs);
-      (void)hipHostFree(s.h_poses);
-      (void)hipHostFree(s.h_simp);
-      (void)hipHostFree(s.h_dist);
-      (void)hipHostFree(s.h_nrm);
+      (void)hipHostFree(s.h_in);
+      (void)hipHostFree(s.h_out);
       (void)hipHostFree(s.h_f64);
       if (s.done) (void)hipEventDestroy(s.done);
       if (s.stream) (void)hipStreamDestroy(s.stream);
@@ -126,8 +120,7 @@ struct ogjk_server_s {
 
   static size_t slot_device_bytes(int32_t cap) {
     const size_t n = (size_t)cap;
-    return n * (8 + 56 + sizeof(gkSimplex) + 4 + 12 + 8 * OGJK_F64_RECORD) +
-           ogjk::workspace_bytes(cap);
+    return n * (kInBytes + kOutBytes + 8 * OGJK_F64_RECORD) + ogjk::workspace_bytes(cap);
   }
 
   hipError_t init_slot(Slot& s) {
@@ -135,18 +128,12 @@ struct ogjk_server_s {
     hipError_t e;
     if ((e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking)) != hipSuccess) return e;
     if ((e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming)) != hipSuccess) return e;
-    if ((e = hipMalloc(&s.d_pairs, 8 * n)) != hipSuccess) return e;
-    if ((e = hipMalloc(&s.d_poses, 56 * n)) != hipSuccess) return e;
-    if ((e = hipMalloc(&s.d_simp, sizeof(gkSimplex) * n)) != hipSuccess) return e;
-    if ((e = hipMalloc(&s.d_dist, 4 * n)) != hipSuccess) return e;
-    if ((e = hipMalloc(&s.d_nrm, 12 * n)) != hipSuccess) return e;
+    if ((e = hipMalloc(&s.d_in, kInBytes * n)) != hipSuccess) return e;
+    if ((e = hipMalloc(&s.d_out, kOutBytes * n)) != hipSuccess) return e;
     if ((e = hipMalloc(&s.d_f64, 8 * OGJK_F64_RECORD * n)) != hipSuccess) return e;
     if ((e = hipMalloc(&s.d_ws, ogjk::workspace_bytes(capacity))) != hipSuccess) return e;
-    if ((e = hipHostMalloc(&s.h_pairs, 8 * n)) != hipSuccess) return e;
-    if ((e = hipHostMalloc(&s.h_poses, 56 * n)) != hipSuccess) return e;
-    if ((e = hipHostMalloc(&s.h_simp, sizeof(gkSimplex) * n)) != hipSuccess) return e;
-    if ((e = hipHostMalloc(&s.h_dist, 4 * n)) != hipSuccess) return e;
-    if ((e = hipHostMalloc(&s.h_nrm, 12 * n)) != hipSuccess) return e;
+    if ((e = hipHostMalloc(&s.h_in, kInBytes * n)) != hipSuccess) return e;
+    if ((e = hipHostMalloc(&s.h_out, kOutBytes * n)) != hipSuccess) return e;
     if ((e = hipHostMalloc(&s.h_f64, 8 * OGJK_F64_RECORD * n)) != hipSuccess) return e;
     return hipSuccess;
   }
@@ -155,38 +142,38 @@ struct ogjk_server_s {
   hipError_t launch(Slot& s) {
     int32_t total = 0;
     s.with_f64 = false;
-    for (size_t k = 0; k < s.reqs.size(); k++) {
-      const Request& r = *s.reqs[k];
-      std::memcpy(s.h_pairs + 2 * (size_t)total, r.pairs, 8 * (size_t)r.n);
-      std::memcpy(s.h_poses + 14 * (size_t)total, r.poses, 56 * (size_t)r.n);
-      s.with_f64 |= r.f64 != nullptr;
-      total += r.n;
+    for (const auto& r : s.reqs) {
+      total += r->n;
+      s.with_f64 |= r->f64 != nullptr;
+    }
+    s.total = total;
+    const size_t t = (size_t)total;
+    int32_t* h_pairs = (int32_t*)s.h_in;
+    float* h_poses = (float*)(s.h_in + 8 * t);
+    int32_t off = 0;
+    for (const auto& r : s.reqs) {
+      std::memcpy(h_pairs + 2 * (size_t)off, r->pairs, 8 * (size_t)r->n);
+      std::memcpy(h_poses + 14 * (size_t)off, r->poses, 56 * (size_t)r->n);
+      off += r->n;
     }
     hipError_t e;
-    if ((e = hipMemcpyAsync(s.d_pairs, s.h_pairs, 8 * (size_t)total, hipMemcpyHostToDevice,
-                            s.stream)) != hipSuccess)
-      return e;
-    if ((e = hipMemcpyAsync(s.d_poses, s.h_poses, 56 * (size_t)total, hipMemcpyHostToDevice,
-                            s.stream)) != hipSuccess)
-      return e;
-    ogjk::SceneArgs a = sa;
-    a.pairs = s.d_pairs;
-    const ogjk::OutArgs o{s.d_simp, s.d_dist, s.d_nrm, s.with_f64 ? s.d_f64 : nullptr};
-    if ((e = ogjk::launch_posed(OGJK_MODE_GJK_EPA, a, s.d_poses, o, total, s.d_ws, s.stream)) !=
+    if ((e = hipMemcpyAsync(s.d_in, s.h_in, kInBytes * t, hipMemcpyHostToDevice, s.stream)) !=
         hipSuccess)
       return e;
-    if ((e = hipMemcpyAsync(s.h_simp, s.d_simp, sizeof(gkSimplex) * (size_t)total,
-                            hipMemcpyDeviceToHost, s.stream)) != hipSuccess)
+    ogjk::SceneArgs a = sa;
+    a.pairs = (const int32_t*)s.d_in;
+    const ogjk::OutArgs o{(gkSimplex*)s.d_out, (float*)(s.d_out + sizeof(gkSimplex) * t),
+                          (float*)(s.d_out + (sizeof(gkSimplex) + 4) * t),
+                          s.with_f64 ? s.d_f64 : nullptr};
+    if ((e = ogjk::launch_posed(OGJK_MODE_GJK_EPA, a, (const float*)(s.d_in + 8 * t), o, total,
+                                s.d_ws, s.stream)) != hipSuccess)
       return e;
-    if ((e = hipMemcpyAsync(s.h_dist, s.d_dist, 4 * (size_t)total, hipMemcpyDeviceToHost,
-                            s.stream)) != hipSuccess)
-      return e;
-    if ((e = hipMemcpyAsync(s.h_nrm, s.d_nrm, 12 * (size_t)total, hipMemcpyDeviceToHost,
-                            s.stream)) != hipSuccess)
+    if ((e = hipMemcpyAsync(s.h_out, s.d_out, kOutBytes * t, hipMemcpyDeviceToHost, s.stream)) !=
+        hipSuccess)
       return e;
     if (s.with_f64 &&
-        (e = hipMemcpyAsync(s.h_f64, s.d_f64, 8 * OGJK_F64_RECORD * (size_t)total,
-                            hipMemcpyDeviceToHost, s.stream)) != hipSuccess)
+        (e = hipMemcpyAsync(s.h_f64, s.d_f64, 8 * OGJK_F64_RECORD * t, hipMemcpyDeviceToHost,
+                            s.stream)) != hipSuccess)
       return e;
     return hipEventRecord(s.done, s.stream);
   }
@@ -265,9 +252,13 @@ struct ogjk_server_s {
           Request& r = *s.reqs[k];
           const size_t o = (size_t)s.offs[k], n = (size_t)r.n;
           if (q == hipSuccess) {
-            if (r.simplices) std::memcpy(r.simplices, s.h_simp + o, sizeof(gkSimplex) * n);
-            if (r.distances) std::memcpy(r.distances, s.h_dist + o, 4 * n);
-            if (r.normals) std::memcpy(r.normals, s.h_nrm + 3 * o, 12 * n);
+            const size_t t = (size_t)s.total;
+            const gkSimplex* h_simp = (const gkSimplex*)s.h_out;
+            const float* h_dist = (const float*)(s.h_out + sizeof(gkSimplex) * t);
+            const float* h_nrm = (const float*)(s.h_out + (sizeof(gkSimplex) + 4) * t);
+            if (r.simplices) std::memcpy(r.simplices, h_simp + o, sizeof(gkSimplex) * n);
+            if (r.distances) std::memcpy(r.distances, h_dist + o, 4 * n);
+            if (r.normals) std::memcpy(r.normals, h_nrm + 3 * o, 12 * n);
             if (r.f64) std::memcpy(r.f64, s.h_f64 + OGJK_F64_RECORD * o, 8 * OGJK_F64_RECORD * n);
           }
         }
