@@ -200,6 +200,51 @@ ogjk_status ogjk_server_info(ogjk_server server, int32_t* slots, int32_t* capaci
                              int64_t* launches, int64_t* pairs_done);
 ogjk_status ogjk_server_destroy(ogjk_server server);
 
+/* ------------------------------------------------------------- broad phase */
+
+/* Uniform grid: grid_size^3 cubic cells of edge cell_size from -boundary on
+ * each axis; an object's cell is clamp(floor((p + boundary) / cell_size), 0,
+ * grid_size - 1) per axis (NaN -> 0).  A cell lists at most max_per_cell
+ * objects, the lowest indices (0: no limit; the reference's grid keeps 512,
+ * visualization/sim_config.h:16, chosen by arrival order).  Cell lists, the
+ * start table and the sort are in the caller's workspace, sized by
+ * ogjk_broad_phase_workspace (needs a device: the sort's size query does). */
+typedef struct ogjk_grid {
+  float cell_size;
+  float boundary;
+  int32_t grid_size;
+  int32_t max_per_cell;
+} ogjk_grid;
+ogjk_status ogjk_broad_phase_workspace(int32_t n_objects, const ogjk_grid* grid, size_t* bytes);
+
+/* Spheres d_spheres[4i..4i+3] = centre x, y, z and bounding radius of object
+ * i.  Writes every pair (i, j), i < j, in the same or a neighbouring cell with
+ * ((dx*dx + dy*dy) + dz*dz) < (r_i + r_j)^2 in FP32: ordered by i, then by
+ * neighbour cell (z, then y, then x, each from -1 to +1), then by j.  The
+ * first max_pairs are written; *n_pairs = how many, *n_found = all found.
+ * Synchronous on stream (the total is read back).  Replaces sim_broad_phase's
+ * grid pass: insert_objects_kernel, count_pairs_kernel, the prefix scan and
+ * generate_pairs_kernel (visualization/integrate_final_gjk.cu:467-570,
+ * 916-956). */
+ogjk_status ogjk_broad_phase(const float* d_spheres, int32_t n_objects, const ogjk_grid* grid,
+                             gkCollisionPair* d_pairs, int32_t max_pairs, int32_t* n_pairs,
+                             int64_t* n_found, void* d_workspace, size_t workspace_bytes,
+                             void* stream);
+
+/* Body pairs -> sub-mesh pairs: body pair p = (a, b) becomes
+ * (d_sub_off[a] + u, d_sub_off[b] + v) for u < d_sub_cnt[a], v < d_sub_cnt[b],
+ * u-major, in body-pair order; a pair naming a body outside [0, n_bodies)
+ * expands to nothing.  The first max_sub_pairs are written; synchronous.
+ * Replaces count_submesh_pairs_kernel, its scan and
+ * expand_submesh_pairs_kernel (visualization/integrate_final_gjk.cu:265-302,
+ * 958-1000). */
+ogjk_status ogjk_expand_pairs_workspace(int32_t n_body_pairs, size_t* bytes);
+ogjk_status ogjk_expand_pairs(const gkCollisionPair* d_body_pairs, int32_t n_body_pairs,
+                              const int32_t* d_sub_off, const int32_t* d_sub_cnt,
+                              int32_t n_bodies, gkCollisionPair* d_sub_pairs,
+                              int32_t max_sub_pairs, int32_t* n_sub_pairs, int64_t* n_found,
+                              void* d_workspace, size_t workspace_bytes, void* stream);
+
 /* Slots a cluster build needs for a scene of n_verts vertices in n_bodies
  * bodies (padding to 32 per body included). */
 int64_t ogjk_cluster_slots(int64_t n_verts, int32_t n_bodies);

@@ -1,0 +1,382 @@
+// Broad phase: uniform-grid sphere overlap -> gkCollisionPair lists, and the
+// body-pair -> sub-mesh-pair expansion, on the device.
+//
+// Same pair set and order as the reference's grid
+// (visualization/integrate_final_gjk.cu:467-570, 916-1002, 265-302), built the
+// other way round: instead of per-cell slot arrays filled by atomics, objects
+// are radix-sorted by cell (stable, so each cell lists its objects in
+// ascending index, one of the orders the reference's atomics can produce) and
+// a start table gives each cell's range.  Threads walk objects in cell order,
+// so the lanes of a wave query the same neighbour cells and their loads
+// coalesce or hit in cache; the three x-neighbours of a row are one contiguous
+// range when cells are not capped.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <stdint.h>
+
+#include <cmath>
+
+#include "ogjk_internal.hpp"
+#include "opengjk/ogjk.h"
+
+ogjk_status ogjk_fail(ogjk_status code, const char* fmt, ...);
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int64_t kMaxCells = int64_t(1) << 30;
+
+__device__ __forceinline__ int cell_axis(float p, float boundary, float cell, int g) {
+  const float f = floorf((p + boundary) / cell);
+  if (!(f >= 0.0f)) return 0;  // negative or NaN
+  if (f > (float)(g - 1)) return g - 1;
+  return (int)f;
+}
+
+__device__ __forceinline__ bool overlap(const float4 a, const float4 b) {
+  const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+  const float d2 = (dx * dx + dy * dy) + dz * dz;
+  const float rs = a.w + b.w;
+  return d2 < rs * rs;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bp_keys(const float4* __restrict__ sph, int n,
+                                                    float cell, float boundary, int g,
+                                                    uint32_t* __restrict__ keys,
+                                                    int32_t* __restrict__ ids) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float4 s = sph[i];
+  const uint32_t cx = cell_axis(s.x, boundary, cell, g);
+  const uint32_t cy = cell_axis(s.y, boundary, cell, g);
+  const uint32_t cz = cell_axis(s.z, boundary, cell, g);
+  keys[i] = cx + (uint32_t)g * (cy + (uint32_t)g * cz);
+  ids[i] = i;
+}
+
+// start[c] = first sorted position whose cell is >= c, for c in [0, cells].
+__global__ __launch_bounds__(kBlock) void k_bp_cell_start(const uint32_t* __restrict__ keys,
+                                                          int n, int64_t cells,
+                                                          int32_t* __restrict__ start) {
+  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (c > cells) return;
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if ((int64_t)keys[mid] < c) lo = mid + 1; else hi = mid;
+  }
+  start[c] = lo;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bp_gather(const float4* __restrict__ sph,
+                                                      const int32_t* __restrict__ ids, int n,
+                                                      float4* __restrict__ sorted) {
+  const int s = blockIdx.x * kBlock + threadIdx.x;
+  if (s < n) sorted[s] = sph[ids[s]];
+}
+
+// One thread per object, in cell order.  Count pass: counts[o] = pairs (o, j),
+// j > o (counts[n] = 0 so the exclusive scan's last entry is the total).
+// Emit pass: writes them at offsets[o], in neighbour-cell order (z, y, x from
+// -1 to +1), ascending j within a cell; positions at or past max_pairs are
+// dropped.
+template <bool EMIT>
+__global__ __launch_bounds__(kBlock) void k_bp_pairs(
+    const float4* __restrict__ sorted, const int32_t* __restrict__ ids,
+    const uint32_t* __restrict__ keys, const int32_t* __restrict__ start, int n, int g, int cap,
+    int64_t* __restrict__ counts, const int64_t* __restrict__ offsets,
+    gkCollisionPair* __restrict__ pairs, int64_t max_pairs) {
+  const int s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= n) {
+    if (!EMIT && s == n) counts[n] = 0;
+    return;
+  }
+  const float4 p = sorted[s];
+  const int o = ids[s];
+  const uint32_t k = keys[s];
+  const int cx = (int)(k % (uint32_t)g);
+  const int cy = (int)((k / (uint32_t)g) % (uint32_t)g);
+  const int cz = (int)(k / ((uint32_t)g * (uint32_t)g));
+  const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx < g - 1 ? cx + 1 : g - 1;
+  int64_t w = EMIT ? offsets[o] : 0;
+  int64_t cnt = 0;
+  for (int dz = -1; dz <= 1; dz++) {
+    const int nz = cz + dz;
+    if (nz < 0 || nz >= g) continue;
+    for (int dy = -1; dy <= 1; dy++) {
+      const int ny = cy + dy;
+      if (ny < 0 || ny >= g) continue;
+      const int64_t row = (int64_t)g * (ny + (int64_t)g * nz);
+      for (int nx = cap > 0 ? x0 : x1; nx <= x1; nx++) {
+        // uncapped: the row's x-neighbours are one contiguous range
+        int lo = start[row + (cap > 0 ? nx : x0)];
+        int hi = start[row + nx + 1];
+        if (cap > 0 && hi - lo > cap) hi = lo + cap;
+        for (int t = lo; t < hi; t++) {
+          const int j = ids[t];
+          if (j <= o) continue;
+          if (!overlap(p, sorted[t])) continue;
+          if (EMIT) {
+            if (w < max_pairs) pairs[w] = gkCollisionPair{o, j};
+            w++;
+          } else {
+            cnt++;
+          }
+        }
+      }
+    }
+  }
+  if (!EMIT) counts[o] = cnt;
+}
+
+// Sub-mesh expansion: counts[p] = sub_cnt[a] * sub_cnt[b] (0 for an index out
+// of [0, n_bodies)), counts[n] = 0.
+__global__ __launch_bounds__(kBlock) void k_sub_counts(const gkCollisionPair* __restrict__ bp,
+                                                       int n, const int32_t* __restrict__ sub_cnt,
+                                                       int nb, int64_t* __restrict__ counts) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p > n) return;
+  if (p == n) {
+    counts[n] = 0;
+    return;
+  }
+  const gkCollisionPair q = bp[p];
+  int64_t c = 0;
+  if (q.idx1 >= 0 && q.idx1 < nb && q.idx2 >= 0 && q.idx2 < nb) {
+    const int64_t ca = sub_cnt[q.idx1], cb = sub_cnt[q.idx2];
+    if (ca > 0 && cb > 0) c = ca * cb;
+  }
+  counts[p] = c;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sub_expand(
+    const gkCollisionPair* __restrict__ bp, int n, const int32_t* __restrict__ sub_off,
+    const int32_t* __restrict__ sub_cnt, const int64_t* __restrict__ counts,
+    const int64_t* __restrict__ offsets, gkCollisionPair* __restrict__ out, int64_t max_out) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= n) return;
+  const int64_t c = counts[p];
+  if (c == 0) return;
+  const gkCollisionPair q = bp[p];
+  const int oa = sub_off[q.idx1], ob = sub_off[q.idx2];
+  const int ca = sub_cnt[q.idx1], cb = sub_cnt[q.idx2];
+  int64_t w = offsets[p];
+  for (int u = 0; u < ca && w < max_out; u++)
+    for (int v = 0; v < cb && w < max_out; v++) out[w++] = gkCollisionPair{oa + u, ob + v};
+}
+
+inline size_t up256(size_t b) { return (b + 255) & ~size_t(255); }
+
+int key_bits(int64_t cells) {
+  int b = 1;
+  while ((int64_t(1) << b) < cells) b++;
+  return b;
+}
+
+// Workspace of the broad phase for n objects over `cells` cells.
+struct BpLayout {
+  size_t keys_in, ids_in, keys_out, ids_out, sorted, counts, offsets, start, temp, total;
+  size_t temp_bytes;
+};
+
+hipError_t bp_layout(int n, int64_t cells, BpLayout* L) {
+  size_t sort_b = 0, scan_b = 0;
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(
+      nullptr, sort_b, (const uint32_t*)nullptr, (uint32_t*)nullptr, (const int32_t*)nullptr,
+      (int32_t*)nullptr, n, 0, key_bits(cells));
+  if (e != hipSuccess) return e;
+  e = hipcub::DeviceScan::ExclusiveSum(nullptr, scan_b, (const int64_t*)nullptr,
+                                       (int64_t*)nullptr, n + 1);
+  if (e != hipSuccess) return e;
+  L->temp_bytes = sort_b > scan_b ? sort_b : scan_b;
+  size_t o = 0;
+  L->keys_in = o;  o += up256(sizeof(uint32_t) * (size_t)n);
+  L->ids_in = o;   o += up256(sizeof(int32_t) * (size_t)n);
+  L->keys_out = o; o += up256(sizeof(uint32_t) * (size_t)n);
+  L->ids_out = o;  o += up256(sizeof(int32_t) * (size_t)n);
+  L->sorted = o;   o += up256(sizeof(float4) * (size_t)n);
+  L->counts = o;   o += up256(sizeof(int64_t) * ((size_t)n + 1));
+  L->offsets = o;  o += up256(sizeof(int64_t) * ((size_t)n + 1));
+  L->start = o;    o += up256(sizeof(int32_t) * ((size_t)cells + 1));
+  L->temp = o;     o += up256(L->temp_bytes);
+  L->total = o;
+  return hipSuccess;
+}
+
+struct ExLayout {
+  size_t counts, offsets, temp, total, temp_bytes;
+};
+
+hipError_t ex_layout(int n, ExLayout* L) {
+  size_t scan_b = 0;
+  const hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, scan_b, (const int64_t*)nullptr,
+                                                        (int64_t*)nullptr, n + 1);
+  if (e != hipSuccess) return e;
+  L->temp_bytes = scan_b;
+  size_t o = 0;
+  L->counts = o;  o += up256(sizeof(int64_t) * ((size_t)n + 1));
+  L->offsets = o; o += up256(sizeof(int64_t) * ((size_t)n + 1));
+  L->temp = o;    o += up256(scan_b);
+  L->total = o;
+  return hipSuccess;
+}
+
+ogjk_status hip_err(hipError_t e, const char* what) {
+  return ogjk_fail(e == hipErrorOutOfMemory ? OGJK_ERR_NOMEM : OGJK_ERR_HIP, "%s: %s", what,
+                   hipGetErrorString(e));
+}
+
+#define BP_HIP(call, what)                              \
+  do {                                                  \
+    const hipError_t e_ = (call);                       \
+    if (e_ != hipSuccess) return hip_err(e_, what);     \
+  } while (0)
+
+ogjk_status check_grid(const ogjk_grid* g, int64_t* cells) {
+  if (!g) return ogjk_fail(OGJK_ERR_INVALID, "NULL grid");
+  if (!(g->cell_size > 0.0f) || !std::isfinite(g->cell_size) || !std::isfinite(g->boundary))
+    return ogjk_fail(OGJK_ERR_INVALID, "cell_size must be finite and > 0, boundary finite");
+  if (g->grid_size < 1) return ogjk_fail(OGJK_ERR_INVALID, "grid_size %d < 1", g->grid_size);
+  if (g->max_per_cell < 0) return ogjk_fail(OGJK_ERR_INVALID, "max_per_cell < 0");
+  const int64_t c = (int64_t)g->grid_size * g->grid_size * g->grid_size;
+  if (c > kMaxCells)
+    return ogjk_fail(OGJK_ERR_UNSUPPORTED, "grid of %lld cells exceeds 2^30", (long long)c);
+  *cells = c;
+  return OGJK_OK;
+}
+
+unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+extern "C" {
+
+ogjk_status ogjk_broad_phase_workspace(int32_t n_objects, const ogjk_grid* grid, size_t* bytes) {
+  if (!bytes) return ogjk_fail(OGJK_ERR_INVALID, "NULL bytes");
+  if (n_objects < 0) return ogjk_fail(OGJK_ERR_INVALID, "n_objects < 0");
+  int64_t cells = 0;
+  const ogjk_status st = check_grid(grid, &cells);
+  if (st != OGJK_OK) return st;
+  BpLayout L;
+  BP_HIP(bp_layout(n_objects, cells, &L), "workspace query");
+  *bytes = L.total;
+  return OGJK_OK;
+}
+
+ogjk_status ogjk_broad_phase(const float* d_spheres, int32_t n_objects, const ogjk_grid* grid,
+                             gkCollisionPair* d_pairs, int32_t max_pairs, int32_t* n_pairs,
+                             int64_t* n_found, void* d_workspace, size_t workspace_bytes,
+                             void* stream) {
+  if (n_pairs) *n_pairs = 0;
+  if (n_found) *n_found = 0;
+  if (n_objects < 0 || max_pairs < 0)
+    return ogjk_fail(OGJK_ERR_INVALID, "n_objects and max_pairs must be >= 0");
+  int64_t cells = 0;
+  const ogjk_status st = check_grid(grid, &cells);
+  if (st != OGJK_OK) return st;
+  if (n_objects == 0) return OGJK_OK;
+  if (!d_spheres || (max_pairs > 0 && !d_pairs))
+    return ogjk_fail(OGJK_ERR_INVALID, "NULL spheres or pairs");
+  BpLayout L;
+  BP_HIP(bp_layout(n_objects, cells, &L), "workspace query");
+  if (!d_workspace || workspace_bytes < L.total)
+    return ogjk_fail(OGJK_ERR_INVALID, "workspace of %zu bytes needed", L.total);
+  char* ws = (char*)d_workspace;
+  auto* keys_in = (uint32_t*)(ws + L.keys_in);
+  auto* ids_in = (int32_t*)(ws + L.ids_in);
+  auto* keys = (uint32_t*)(ws + L.keys_out);
+  auto* ids = (int32_t*)(ws + L.ids_out);
+  auto* sorted = (float4*)(ws + L.sorted);
+  auto* counts = (int64_t*)(ws + L.counts);
+  auto* offsets = (int64_t*)(ws + L.offsets);
+  auto* start = (int32_t*)(ws + L.start);
+  void* temp = ws + L.temp;
+  const hipStream_t st_ = (hipStream_t)stream;
+  const int n = n_objects, g = grid->grid_size;
+  const float4* sph = (const float4*)d_spheres;
+
+  hipLaunchKernelGGL(k_bp_keys, dim3(blocks_for(n)), dim3(kBlock), 0, st_, sph, n,
+                     grid->cell_size, grid->boundary, g, keys_in, ids_in);
+  BP_HIP(hipGetLastError(), "cell keys");
+  size_t tb = L.temp_bytes;
+  BP_HIP(hipcub::DeviceRadixSort::SortPairs(temp, tb, keys_in, keys, ids_in, ids, n, 0,
+                                            key_bits(cells), st_),
+         "cell sort");
+  hipLaunchKernelGGL(k_bp_cell_start, dim3(blocks_for(cells + 1)), dim3(kBlock), 0, st_, keys,
+                     n, cells, start);
+  hipLaunchKernelGGL(k_bp_gather, dim3(blocks_for(n)), dim3(kBlock), 0, st_, sph, ids, n,
+                     sorted);
+  hipLaunchKernelGGL(k_bp_pairs<false>, dim3(blocks_for((int64_t)n + 1)), dim3(kBlock), 0, st_,
+                     sorted, ids, keys, start, n, g, grid->max_per_cell, counts,
+                     (const int64_t*)nullptr, (gkCollisionPair*)nullptr, (int64_t)0);
+  BP_HIP(hipGetLastError(), "pair count");
+  tb = L.temp_bytes;
+  BP_HIP(hipcub::DeviceScan::ExclusiveSum(temp, tb, counts, offsets, n + 1, st_), "pair scan");
+  if (max_pairs > 0) {
+    hipLaunchKernelGGL(k_bp_pairs<true>, dim3(blocks_for(n)), dim3(kBlock), 0, st_, sorted, ids,
+                       keys, start, n, g, grid->max_per_cell, counts, offsets, d_pairs,
+                       (int64_t)max_pairs);
+    BP_HIP(hipGetLastError(), "pair emit");
+  }
+  int64_t total = 0;
+  BP_HIP(hipMemcpyAsync(&total, offsets + n, sizeof(int64_t), hipMemcpyDeviceToHost, st_),
+         "pair total");
+  BP_HIP(hipStreamSynchronize(st_), "broad phase");
+  if (n_found) *n_found = total;
+  if (n_pairs) *n_pairs = (int32_t)(total < max_pairs ? total : max_pairs);
+  return OGJK_OK;
+}
+
+ogjk_status ogjk_expand_pairs_workspace(int32_t n_body_pairs, size_t* bytes) {
+  if (!bytes) return ogjk_fail(OGJK_ERR_INVALID, "NULL bytes");
+  if (n_body_pairs < 0) return ogjk_fail(OGJK_ERR_INVALID, "n_body_pairs < 0");
+  ExLayout L;
+  BP_HIP(ex_layout(n_body_pairs, &L), "workspace query");
+  *bytes = L.total;
+  return OGJK_OK;
+}
+
+ogjk_status ogjk_expand_pairs(const gkCollisionPair* d_body_pairs, int32_t n_body_pairs,
+                              const int32_t* d_sub_off, const int32_t* d_sub_cnt,
+                              int32_t n_bodies, gkCollisionPair* d_sub_pairs,
+                              int32_t max_sub_pairs, int32_t* n_sub_pairs, int64_t* n_found,
+                              void* d_workspace, size_t workspace_bytes, void* stream) {
+  if (n_sub_pairs) *n_sub_pairs = 0;
+  if (n_found) *n_found = 0;
+  if (n_body_pairs < 0 || max_sub_pairs < 0 || n_bodies < 0)
+    return ogjk_fail(OGJK_ERR_INVALID, "counts must be >= 0");
+  if (n_body_pairs == 0) return OGJK_OK;
+  if (!d_body_pairs || !d_sub_off || !d_sub_cnt || (max_sub_pairs > 0 && !d_sub_pairs))
+    return ogjk_fail(OGJK_ERR_INVALID, "NULL input or output");
+  ExLayout L;
+  BP_HIP(ex_layout(n_body_pairs, &L), "workspace query");
+  if (!d_workspace || workspace_bytes < L.total)
+    return ogjk_fail(OGJK_ERR_INVALID, "workspace of %zu bytes needed", L.total);
+  char* ws = (char*)d_workspace;
+  auto* counts = (int64_t*)(ws + L.counts);
+  auto* offsets = (int64_t*)(ws + L.offsets);
+  const hipStream_t st_ = (hipStream_t)stream;
+  const int n = n_body_pairs;
+  hipLaunchKernelGGL(k_sub_counts, dim3(blocks_for((int64_t)n + 1)), dim3(kBlock), 0, st_,
+                     d_body_pairs, n, d_sub_cnt, n_bodies, counts);
+  BP_HIP(hipGetLastError(), "sub-mesh counts");
+  size_t tb = L.temp_bytes;
+  BP_HIP(hipcub::DeviceScan::ExclusiveSum(ws + L.temp, tb, counts, offsets, n + 1, st_),
+         "sub-mesh scan");
+  if (max_sub_pairs > 0) {
+    hipLaunchKernelGGL(k_sub_expand, dim3(blocks_for(n)), dim3(kBlock), 0, st_, d_body_pairs, n,
+                       d_sub_off, d_sub_cnt, counts, offsets, d_sub_pairs,
+                       (int64_t)max_sub_pairs);
+    BP_HIP(hipGetLastError(), "sub-mesh expand");
+  }
+  int64_t total = 0;
+  BP_HIP(hipMemcpyAsync(&total, offsets + n, sizeof(int64_t), hipMemcpyDeviceToHost, st_),
+         "sub-mesh total");
+  BP_HIP(hipStreamSynchronize(st_), "sub-mesh expansion");
+  if (n_found) *n_found = total;
+  if (n_sub_pairs) *n_sub_pairs = (int32_t)(total < max_sub_pairs ? total : max_sub_pairs);
+  return OGJK_OK;
+}
+
+}  // extern "C"

@@ -41,7 +41,8 @@ C_ABI_SYMBOLS = [
     "ogjk_collide_host_indexed", "ogjk_cluster_slots", "ogjk_cluster_supers", "ogjk_build_clusters",
     "ogjk_graph_capture_scene", "ogjk_graph_launch", "ogjk_graph_destroy", "ogjk_collide_posed",
     "ogjk_server_create", "ogjk_server_submit", "ogjk_server_wait", "ogjk_server_poll",
-    "ogjk_server_info", "ogjk_server_destroy",
+    "ogjk_server_info", "ogjk_server_destroy", "ogjk_broad_phase_workspace", "ogjk_broad_phase",
+    "ogjk_expand_pairs_workspace", "ogjk_expand_pairs",
 ]
 
 
@@ -65,6 +66,11 @@ class _Scene(C.Structure):
 
 class _ServerConfig(C.Structure):
     _fields_ = [("max_batch_pairs", C.c_int32), ("slots", C.c_int32), ("memory_budget", C.c_int64)]
+
+
+class _Grid(C.Structure):
+    _fields_ = [("cell_size", C.c_float), ("boundary", C.c_float), ("grid_size", C.c_int32),
+                ("max_per_cell", C.c_int32)]
 
 
 class _Scene64(C.Structure):
@@ -118,6 +124,13 @@ def lib():
             "ogjk_collide_polytopes_indexed": (C.c_int, [C.c_int, vp, i32, vp, i32, vp, vp, vp, vp]),
             "ogjk_collide_host": (C.c_int, [C.c_int, i32, vp, vp, vp, vp, vp]),
             "ogjk_collide_host_indexed": (C.c_int, [C.c_int, i32, i32, vp, vp, vp, vp, vp]),
+            "ogjk_broad_phase_workspace": (C.c_int, [i32, C.POINTER(_Grid),
+                                                     C.POINTER(C.c_size_t)]),
+            "ogjk_broad_phase": (C.c_int, [vp, i32, C.POINTER(_Grid), vp, i32, C.POINTER(i32),
+                                           C.POINTER(i64), vp, C.c_size_t, vp]),
+            "ogjk_expand_pairs_workspace": (C.c_int, [i32, C.POINTER(C.c_size_t)]),
+            "ogjk_expand_pairs": (C.c_int, [vp, i32, vp, vp, i32, vp, i32, C.POINTER(i32),
+                                            C.POINTER(i64), vp, C.c_size_t, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -412,3 +425,106 @@ class Graph:
 def simplices_from_device(t) -> np.ndarray:
     """(n, 27) int32 device tensor of gkSimplex records -> numpy records."""
     return np.ascontiguousarray(t.cpu().numpy()).view(SIMPLEX_DTYPE).reshape(-1)
+
+
+# ------------------------------------------------------------------ broad phase
+
+class Grid:
+    """Uniform grid of the broad phase (ogjk_grid): grid_size^3 cells of edge
+    cell_size from -boundary; max_per_cell 0 lists every object of a cell."""
+
+    def __init__(self, cell_size: float, boundary: float, grid_size: int, max_per_cell: int = 0):
+        self.cell_size, self.boundary = float(cell_size), float(boundary)
+        self.grid_size, self.max_per_cell = int(grid_size), int(max_per_cell)
+
+    @classmethod
+    def for_spheres(cls, max_radius: float, boundary: float, max_grid: int = 30,
+                    max_per_cell: int = 0) -> "Grid":
+        """The reference's sizing (visualization/integrate_final_gjk.cu:813-819):
+        cells at least one largest diameter wide, at most max_grid per axis,
+        computed in FP32."""
+        f = np.float32
+        cell = max(f(2.0) * f(max_radius), f(2.0) * f(boundary) / f(max_grid))
+        g = max(1, min(int(f(2.0) * f(boundary) / f(cell)), max_grid))
+        return cls(float(cell), boundary, g, max_per_cell)
+
+    def _c(self) -> _Grid:
+        return _Grid(self.cell_size, self.boundary, self.grid_size, self.max_per_cell)
+
+    def __repr__(self):
+        return (f"Grid(cell_size={self.cell_size}, boundary={self.boundary}, "
+                f"grid_size={self.grid_size}, max_per_cell={self.max_per_cell})")
+
+
+def _torch_stream(stream):
+    import torch
+    if stream is None:
+        return torch.cuda.current_stream().cuda_stream
+    return stream.cuda_stream if hasattr(stream, "cuda_stream") else stream
+
+
+class BroadPhase:
+    """Device broad phase over n spheres with a fixed grid: owns its workspace
+    and a pair buffer of max_pairs (default 50 per object, the reference's
+    allocation, visualization/integrate_final_gjk.cu:730)."""
+
+    def __init__(self, n_objects: int, grid: Grid, max_pairs: int | None = None, device=None):
+        import torch
+        self.torch = torch
+        self.n, self.grid = int(n_objects), grid
+        self.max_pairs = int(50 * self.n if max_pairs is None else max_pairs)
+        dev = device or torch.device("cuda", torch.cuda.current_device())
+        b = C.c_size_t()
+        _check(lib().ogjk_broad_phase_workspace(self.n, C.byref(grid._c()), C.byref(b)))
+        self.workspace = torch.empty(max(int(b.value), 1), dtype=torch.uint8, device=dev)
+        self.pairs = torch.empty((max(self.max_pairs, 1), 2), dtype=torch.int32, device=dev)
+
+    def __call__(self, spheres, stream=None):
+        """spheres: (n, 4) float32 device tensor.  Returns (pairs (m, 2) view
+        of the pair buffer, found); m = min(found, max_pairs)."""
+        if spheres.shape != (self.n, 4) or spheres.dtype != self.torch.float32:
+            raise OgjkError(f"spheres must be float32 of shape ({self.n}, 4)")
+        if not spheres.is_contiguous():
+            raise OgjkError("spheres must be contiguous")
+        m, found = C.c_int32(), C.c_int64()
+        _check(lib().ogjk_broad_phase(
+            spheres.data_ptr(), self.n, C.byref(self.grid._c()), self.pairs.data_ptr(),
+            self.max_pairs, C.byref(m), C.byref(found), self.workspace.data_ptr(),
+            self.workspace.numel(), _torch_stream(stream)))
+        return self.pairs[:m.value], found.value
+
+
+def broad_phase(spheres, grid: Grid, max_pairs: int | None = None, stream=None):
+    """One-shot device broad phase; returns (pairs (m, 2) int32 tensor, found)."""
+    bp = BroadPhase(spheres.shape[0], grid, max_pairs, device=spheres.device)
+    pairs, found = bp(spheres, stream)
+    return pairs.clone(), found
+
+
+def expand_pairs(body_pairs, sub_off, sub_cnt, max_sub_pairs: int | None = None, stream=None):
+    """Body pairs -> sub-mesh pairs on the device (ogjk_expand_pairs).
+    body_pairs (m, 2) int32, sub_off / sub_cnt int32 per body, all device
+    tensors.  max_sub_pairs defaults to the exact count (computed on the host).
+    Returns (sub pairs (k, 2) int32 tensor, found)."""
+    import torch
+    n = int(body_pairs.shape[0])
+    nb = int(sub_cnt.shape[0])
+    if max_sub_pairs is None:
+        if n:
+            bpi = body_pairs.long()
+            ok = (bpi >= 0).all(1) & (bpi < nb).all(1)
+            cnt = sub_cnt.long().clamp(min=0)
+            bpc = bpi.clamp(0, max(nb - 1, 0))
+            max_sub_pairs = int((cnt[bpc[:, 0]] * cnt[bpc[:, 1]] * ok).sum().item())
+        else:
+            max_sub_pairs = 0
+    b = C.c_size_t()
+    _check(lib().ogjk_expand_pairs_workspace(n, C.byref(b)))
+    ws = torch.empty(max(int(b.value), 1), dtype=torch.uint8, device=body_pairs.device)
+    out = torch.empty((max(max_sub_pairs, 1), 2), dtype=torch.int32, device=body_pairs.device)
+    m, found = C.c_int32(), C.c_int64()
+    _check(lib().ogjk_expand_pairs(
+        body_pairs.data_ptr(), n, sub_off.data_ptr(), sub_cnt.data_ptr(), nb, out.data_ptr(),
+        max_sub_pairs, C.byref(m), C.byref(found), ws.data_ptr(), ws.numel(),
+        _torch_stream(stream)))
+    return out[:m.value], found.value

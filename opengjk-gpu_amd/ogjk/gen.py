@@ -106,3 +106,17 @@ def posed_requests(n_bodies: int, n_requests: int, seed: int = 4, kmin: int = 1,
         poses = np.concatenate([q, t], axis=2).reshape(k, 14).astype(np.float32)
         out.append((np.stack([i, j], 1).astype(np.int32), poses))
     return out
+
+
+def spheres(n: int, seed: int = 5, box: float = 10.0, rmin: float = 0.25, rmax: float = 1.0,
+            outside: float = 0.0):
+    """Broad-phase input: (n, 4) float32 spheres, centres U[-box, box]^3 and
+    radii U[rmin, rmax]; a fraction `outside` of the centres is pushed up to
+    one box width past the faces (those clamp to the border cells)."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-box, box, size=(n, 3))
+    if outside > 0:
+        m = rng.random(n) < outside
+        c[m] *= rng.uniform(1.0, 2.0, size=(int(m.sum()), 1))
+    r = rng.uniform(rmin, rmax, size=(n, 1))
+    return np.concatenate([c, r], 1).astype(np.float32)

@@ -209,3 +209,56 @@ def posed_pool(coords, off, cnt, pairs, poses):
                      for k in range(3)], 1)
     return (pool, offs, cnts.astype(np.int32), np.arange(0, 2 * n, 2, dtype=np.int32),
             np.arange(1, 2 * n, 2, dtype=np.int32))
+
+
+# -------------------------------------------------------------- broad phase
+
+LIBBP = os.path.join(_HERE, "lib", "liboracle_bp.so")
+
+
+def _bp_lib():
+    if "bp" not in _libs:
+        if not os.path.exists(LIBBP):
+            raise RuntimeError(f"oracle library missing: {LIBBP} (run __graft_entry__.build())")
+        lib = C.CDLL(LIBBP)
+        lib.oracle_broad_phase.restype = C.c_int64
+        lib.oracle_broad_phase.argtypes = [C.c_void_p, C.c_int32, C.c_float, C.c_float, C.c_int32,
+                                           C.c_int32, C.c_void_p, C.c_int64]
+        lib.oracle_expand_pairs.restype = C.c_int64
+        lib.oracle_expand_pairs.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                            C.c_int32, C.c_void_p, C.c_int64]
+        _libs["bp"] = lib
+    return _libs["bp"]
+
+
+def broad_phase(spheres, cell_size: float, boundary: float, grid_size: int,
+                max_per_cell: int = 0, max_pairs: int | None = None):
+    """Grid broad phase over (n, 4) float32 spheres (x, y, z, radius).
+    Returns (pairs (m, 2) int32 with m = min(found, max_pairs), found)."""
+    s = np.ascontiguousarray(spheres, dtype=np.float32).reshape(-1, 4)
+    lib = _bp_lib()
+    if max_pairs is None:
+        found = lib.oracle_broad_phase(_p(s), len(s), cell_size, boundary, grid_size,
+                                       max_per_cell, None, 0)
+        if found < 0:
+            raise MemoryError("oracle broad phase scratch")
+        max_pairs = found
+    out = np.zeros((max(max_pairs, 1), 2), np.int32)
+    found = lib.oracle_broad_phase(_p(s), len(s), cell_size, boundary, grid_size, max_per_cell,
+                                   _p(out), max_pairs)
+    if found < 0:
+        raise MemoryError("oracle broad phase scratch")
+    return out[:min(found, max_pairs)].copy(), int(found)
+
+
+def expand_pairs(body_pairs, sub_off, sub_cnt, max_sub: int | None = None):
+    """Body pairs -> sub-mesh pairs.  Returns (sub pairs (m, 2) int32, found)."""
+    bp = np.ascontiguousarray(body_pairs, dtype=np.int32).reshape(-1, 2)
+    so = np.ascontiguousarray(sub_off, dtype=np.int32)
+    sc = np.ascontiguousarray(sub_cnt, dtype=np.int32)
+    lib = _bp_lib()
+    if max_sub is None:
+        max_sub = lib.oracle_expand_pairs(_p(bp), len(bp), _p(so), _p(sc), len(sc), None, 0)
+    out = np.zeros((max(max_sub, 1), 2), np.int32)
+    found = lib.oracle_expand_pairs(_p(bp), len(bp), _p(so), _p(sc), len(sc), _p(out), max_sub)
+    return out[:min(found, max_sub)].copy(), int(found)
