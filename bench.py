@@ -10,6 +10,7 @@ Workloads (--workload):
   noisy_sphere_large: 10,000 pairs of 100,000-vertex polytopes, unit-sphere
       points perturbed by N(0, 0.01^2) per coordinate, centres U[-1.5,1.5]^3
       (about 50% penetrate).
+  serving, broadphase, scene_step: see the functions of those names.
 Every pair runs GJK then EPA: FP32 vertex storage, FP32 support sweep with an
 exact FP64 re-rank, FP64 sub-distance and EPA arithmetic; the larger
 workloads sweep large bodies through their vertex clusters.
@@ -286,12 +287,168 @@ def serving_cpu_baseline(scene, b, reqs, max_pairs: int = 200_000):
     return base, parity
 
 
+def _event_steps(torch, fn, steps: int, warmup: int):
+    """Warm up, then run `steps` calls of fn between HIP events on the
+    current stream.  Returns (wall seconds, per-step event ms)."""
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+    t = time.perf_counter()
+    for a, b in evs:
+        a.record(stream)
+        fn()
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    return wall, np.array([a.elapsed_time(b) for a, b in evs])
+
+
+def broadphase(args) -> int:
+    """Broad-phase leg: 1,000,000 spheres (centres U[-100,100]^3, radii
+    U[0.2,1]) on a 100^3 grid of 2.0 cells; a step is one ogjk_broad_phase
+    call (cell keys, radix sort, start table, count, scan, emit, total read
+    back).  Throughput in objects/s."""
+    import torch
+    import oracle
+    import ogjk
+    from ogjk import gen
+    n = args.pairs or 1_000_000
+    box = 100.0 * (n / 1_000_000) ** (1.0 / 3.0)
+    g = max(1, int(box))
+    s = gen.spheres(n, seed=12, box=box, rmin=0.2, rmax=1.0)
+    grid = ogjk.Grid(2.0 * box / g, box, g)
+    d = torch.from_numpy(s).cuda()
+    bp = ogjk.BroadPhase(n, grid, max_pairs=8 * n)
+    pairs, found = bp(d)
+    torch.cuda.synchronize()
+    got = pairs.cpu().numpy()
+    wall, ms = _event_steps(torch, lambda: bp(d), args.steps, args.warmup)
+    ms_step = wall / args.steps * 1e3
+    # algorithmic bytes: every sphere read once (16 B), every pair written once (8 B)
+    alg = 16.0 * n + 8.0 * found
+    achieved = alg / (np.mean(ms) * 1e-3) / 1e9
+    line = {
+        "metric": "broad-phase objects/s (uniform grid -> gkCollisionPair list)",
+        "value": n / (wall / args.steps), "unit": "objects/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"broad phase: {n} spheres, centres U[-{box:.1f},{box:.1f}]^3, radii "
+                               f"U[0.2,1], grid {g}^3 cells of {grid.cell_size:.3f}",
+                   "objects": n, "pairs_found": found},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "whole broad phase (7 launches + total readback)",
+                     "kernel_ms_mean": float(np.mean(ms)), "algorithmic_bytes_per_launch": alg,
+                     "algorithmic_bytes": "16 B per sphere read + 8 B per pair written"},
+    }
+    if not args.no_cpu:
+        t = time.perf_counter()
+        ref, rfound = oracle.broad_phase(s, grid.cell_size, grid.boundary, grid.grid_size)
+        el = time.perf_counter() - t
+        line["cpu_baseline"] = {
+            "value": n / el, "unit": "objects/s", "cores": 1, "kind": "port",
+            "sample": f"the whole workload ({n} spheres), C oracle of the reference grid, one "
+                      f"thread, {el:.2f}s"}
+        line["parity"] = {"pairs_found": found, "oracle_found": rfound,
+                          "identical_list": bool(rfound == found and np.array_equal(got, ref))}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def scene_step(args) -> int:
+    """Scene-step leg: the serving scene's 20,000 bodies (8-512 vertices,
+    log-uniform ellipsoids in local frames), one pose per body (uniform unit
+    quaternion, translation U[-40,40]^3), bounding spheres through the device
+    broad phase (grid of at most 64^3 cells, cells one largest diameter
+    wide), then GJK+EPA on every pair found.  A step is both phases."""
+    import torch
+    import oracle
+    import ogjk
+    from ogjk import gen
+    ogjk.set_config(32, 64, False)  # broad-phase pairs are close: measured best
+    b = gen.make(gen.ELLIPSOID, 10_000, seed=4, box=0.0, nmin=8, nmax=512,
+                 nthreads=min(len(os.sched_getaffinity(0)), 16))
+    nb = len(b.cnt)
+    radii = gen.body_radii(b.coords, b.off, b.cnt)
+    tbox = 40.0
+    poses = gen.body_poses(nb, seed=7, tbox=tbox)
+    grid = ogjk.Grid.for_spheres(float(radii.max()), tbox, 64)
+    sc = ogjk.DeviceScene(b)
+    step = ogjk.SceneStep(sc, radii, grid, max_pairs=20 * nb, f64_record=False)
+    dp = torch.from_numpy(poses).cuda()
+    pairs, found, out = step(dp)
+    torch.cuda.synchronize()
+    got = pairs.cpu().numpy()
+    n = len(got)
+    wall, ms = _event_steps(torch, lambda: step(dp), args.steps, args.warmup)
+    # narrow phase alone on the same pairs, for the kernel roofline
+    ws = step.workspace
+    nwall, nms = _event_steps(torch, lambda: sc.collide_posed_bodies(3, pairs, dp, out=out,
+                                                                     workspace=ws),
+                              args.steps, args.warmup)
+    cnt = b.cnt.astype(np.int64)
+    alg = float(12 * (cnt[got[:, 0]] + cnt[got[:, 1]]).sum() + n * (8 + 56 + 32 + 124))
+    achieved = alg / (np.mean(nms) * 1e-3) / 1e9
+    line = {
+        "metric": "scene step pairs/s (broad phase + posed GJK+EPA on the pairs found)",
+        "value": n / (wall / args.steps), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"scene step: {nb} bodies (8-512 vertices log-uniform), one pose "
+                               f"per body, t ~ U[-{tbox:.0f},{tbox:.0f}]^3, {grid}",
+                   "bodies": nb, "pairs": n, "lanes_per_pair": {"gjk": 32, "epa": 64}},
+        "narrow_phase_ms": float(np.mean(nms)),
+        "broad_phase_ms": float(np.mean(ms) - np.mean(nms)),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"k_gjk + k_epa_list over the {n} pairs of a step",
+                     "kernel_ms_mean": float(np.mean(nms)), "algorithmic_bytes_per_launch": alg,
+                     "algorithmic_bytes": "compulsory: vertices once + pair, poses, record, "
+                                          "results"},
+    }
+    if not args.no_cpu:
+        cores = min(len(os.sched_getaffinity(0)), 16)
+        spheres = np.concatenate([poses[:, 4:7], radii[:, None]], 1)
+        t = time.perf_counter()
+        ref_pairs, rfound = oracle.broad_phase(spheres, grid.cell_size, grid.boundary,
+                                               grid.grid_size)
+        side = np.concatenate([poses[ref_pairs[:, 0]], poses[ref_pairs[:, 1]]], 1)
+        t_bp = time.perf_counter() - t
+        pool, off, cnt2, pa, pb = oracle.posed_pool(b.coords, b.off, b.cnt, ref_pairs, side)
+        t = time.perf_counter()
+        ref = oracle.run(3, pool, off, cnt2, pa, pb, bits=64, nthreads=cores)
+        t_np = time.perf_counter() - t
+        el = t_bp + t_np
+        step_out = step.scene.collide_posed_bodies(3, pairs, dp, f64_record=True)
+        torch.cuda.synchronize()
+        rec = step_out["f64"][:n].cpu().numpy()
+        exact = ((rec[:, 0] == ref.distances) &
+                 (rec[:, 1:4] == ref.simplices["witnesses"][:, 0]).all(1) &
+                 (rec[:, 4:7] == ref.simplices["witnesses"][:, 1]).all(1) &
+                 (rec[:, 7:10] == ref.normals).all(1)) if len(ref_pairs) == n else np.zeros(1)
+        line["cpu_baseline"] = {
+            "value": rfound / el, "unit": "pairs/s", "cores": cores, "kind": "port",
+            "sample": f"one whole step: C oracle broad phase (1 thread, {t_bp:.2f}s) + FP64 "
+                      f"oracle GJK+EPA on the world vertices of the {rfound} pairs ({cores} "
+                      f"threads, {t_np:.2f}s; world transform untimed)"}
+        line["parity"] = {"pairs_found": found, "oracle_found": rfound,
+                          "identical_pair_list": bool(rfound == found and
+                                                      np.array_equal(got, ref_pairs)),
+                          "bit_exact_fraction": float(exact.mean()),
+                          "overlap_fraction": float((ref.distances <= EPS64).mean())}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="sphere64", choices=sorted(WORKLOADS) + ["serving"])
+    ap.add_argument("--workload", default="sphere64", choices=sorted(WORKLOADS) + ["serving", "broadphase", "scene_step"])
     ap.add_argument("--qps", type=int, default=1000, help="serving: request rate")
     ap.add_argument("--requests", type=int, default=2000, help="serving: requests per rate")
     ap.add_argument("--slots", type=int, default=8, help="serving: launches in flight")
@@ -321,6 +478,10 @@ def main() -> int:
         return 1
     if args.workload == "serving":
         return serving(args)
+    if args.workload == "broadphase":
+        return broadphase(args)
+    if args.workload == "scene_step":
+        return scene_step(args)
     from ogjk import autotune
     spec = WORKLOADS[args.workload]
     lg, le = (int(x) for x in args.lanes.split(",")) if args.lanes else spec["lanes"]

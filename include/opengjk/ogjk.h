@@ -171,6 +171,17 @@ ogjk_status ogjk_collide_posed(int mode, const ogjk_scene* scene, const int32_t*
                                float* d_distances, float* d_normals, double* d_f64,
                                void* d_workspace, size_t workspace_bytes, void* stream);
 
+/* As ogjk_collide_posed with one pose per scene body: body b is placed by
+ * d_body_poses[7b..7b+6], and pair i is d_pairs[i] (a broad-phase list, see
+ * ogjk_broad_phase).  Replaces sim_step's transform_to_world_kernel followed
+ * by compute_minimum_distance_indexed_device and compute_epa_indexed_device
+ * (visualization/integrate_final_gjk.cu:304-332, 1016-1035). */
+ogjk_status ogjk_collide_posed_bodies(int mode, const ogjk_scene* scene,
+                                      const gkCollisionPair* d_pairs, const float* d_body_poses,
+                                      int32_t n_pairs, gkSimplex* d_simplices,
+                                      float* d_distances, float* d_normals, double* d_f64,
+                                      void* d_workspace, size_t workspace_bytes, void* stream);
+
 /* Serving loop over a resident scene (no counterpart in the reference, whose
  * calls are synchronous and allocate per call, GJK/gpu/openGJK.cu:2791-2883).
  * Requests of posed pairs (layout of ogjk_collide_posed, host memory) are

@@ -330,7 +330,29 @@ ogjk_status ogjk_collide_posed(int mode, const ogjk_scene* scene, const int32_t*
   const ogjk_status s2 = scene_args(scene, d_pairs, n_pairs, &sa);
   if (s2 != OGJK_OK) return s2;
   const ogjk::OutArgs oa{d_simplices, d_distances, d_normals, d_f64};
-  OGJK_HIP(ogjk::launch_posed(mode, sa, d_poses, oa, n_pairs, d_workspace, (hipStream_t)stream),
+  OGJK_HIP(ogjk::launch_posed(mode, sa, d_poses, false, oa, n_pairs, d_workspace,
+                              (hipStream_t)stream),
+           "kernel launch");
+  return OGJK_OK;
+}
+
+ogjk_status ogjk_collide_posed_bodies(int mode, const ogjk_scene* scene,
+                                      const gkCollisionPair* d_pairs, const float* d_body_poses,
+                                      int32_t n_pairs, gkSimplex* d_simplices,
+                                      float* d_distances, float* d_normals, double* d_f64,
+                                      void* d_workspace, size_t workspace_bytes, void* stream) {
+  const ogjk_status st = check_out(mode, n_pairs, d_simplices, d_distances, d_normals);
+  if (st != OGJK_OK) return st;
+  if (n_pairs == 0) return OGJK_OK;
+  if (!d_pairs || !d_body_poses) return fail(OGJK_ERR_INVALID, "posed pairs need pairs and poses");
+  if (mode != OGJK_MODE_EPA && (!d_workspace || workspace_bytes < ogjk::workspace_bytes(n_pairs)))
+    return fail(OGJK_ERR_INVALID, "workspace of %zu bytes needed", ogjk::workspace_bytes(n_pairs));
+  ogjk::SceneArgs sa;
+  const ogjk_status s2 = scene_args(scene, (const int32_t*)d_pairs, n_pairs, &sa);
+  if (s2 != OGJK_OK) return s2;
+  const ogjk::OutArgs oa{d_simplices, d_distances, d_normals, d_f64};
+  OGJK_HIP(ogjk::launch_posed(mode, sa, d_body_poses, true, oa, n_pairs, d_workspace,
+                              (hipStream_t)stream),
            "kernel launch");
   return OGJK_OK;
 }

@@ -83,8 +83,10 @@ hipError_t launch_poly(int mode, const PolyArgs& p, const OutArgs& o, int n, voi
                        hipStream_t st);
 // Scene bodies under per-pair poses (14 floats per pair: quaternion w, x, y,
 // z and translation of the first body, then of the second); s.pairs required.
-hipError_t launch_posed(int mode, const SceneArgs& s, const float* poses, const OutArgs& o,
-                        int n, void* ws, hipStream_t st);
+// Posed scene bodies: poses per pair side (14 floats per pair) or, by_body,
+// per scene body (7 floats per body).
+hipError_t launch_posed(int mode, const SceneArgs& s, const float* poses, bool by_body,
+                        const OutArgs& o, int n, void* ws, hipStream_t st);
 // Vertex clusters (see ogjk_cluster.hip): slots needed, and the build.
 int64_t cluster_slots(int64_t n_verts, int32_t n_bodies);
 int64_t cluster_supers(int64_t n_slots);

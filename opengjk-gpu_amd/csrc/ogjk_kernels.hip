@@ -90,6 +90,7 @@ struct PoseSrc {
   int32_t nb;
   const int32_t* pairs;
   const float* poses;
+  int32_t by_body;  // poses[7 b] per body b, else poses[14 i], [14 i + 7] per pair side
   using BodyT = PBody;
   static constexpr int kGjkWaves = OGJK_GJK_WAVES_WIDE;
   static constexpr bool kStage = false;
@@ -105,8 +106,10 @@ struct PoseSrc {
     const int a = pairs[2 * i], b = pairs[2 * i + 1];
     if (a < 0 || a >= nb || b < 0 || b >= nb) return false;
     const int64_t oa = off[a], ob = off[b];
-    A = PBody{{x + oa, y + oa, z + oa, cnt[a], r1[a]}, poses + 14 * (int64_t)i};
-    B = PBody{{x + ob, y + ob, z + ob, cnt[b], r1[b]}, poses + 14 * (int64_t)i + 7};
+    const float* pa = by_body ? poses + 7 * (int64_t)a : poses + 14 * (int64_t)i;
+    const float* pb = by_body ? poses + 7 * (int64_t)b : poses + 14 * (int64_t)i + 7;
+    A = PBody{{x + oa, y + oa, z + oa, cnt[a], r1[a]}, pa};
+    B = PBody{{x + ob, y + ob, z + ob, cnt[b], r1[b]}, pb};
     return true;
   }
 };
@@ -820,10 +823,10 @@ hipError_t launch_scene_f64(int mode, const SceneArgs& s, const OutArgs& o, int 
   return launch_any(mode, src, o, n, ws, st);
 }
 
-hipError_t launch_posed(int mode, const SceneArgs& s, const float* poses, const OutArgs& o,
-                        int n, void* ws, hipStream_t st) {
+hipError_t launch_posed(int mode, const SceneArgs& s, const float* poses, bool by_body,
+                        const OutArgs& o, int n, void* ws, hipStream_t st) {
   const PoseSrc src{(const float*)s.x, (const float*)s.y, (const float*)s.z, s.off, s.cnt, s.r1,
-                    s.n_bodies, s.pairs, poses};
+                    s.n_bodies, s.pairs, poses, by_body ? 1 : 0};
   return launch_any(mode, src, o, n, ws, st);
 }
 

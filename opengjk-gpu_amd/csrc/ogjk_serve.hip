@@ -165,7 +165,7 @@ struct ogjk_server_s {
     const ogjk::OutArgs o{(gkSimplex*)s.d_out, (float*)(s.d_out + sizeof(gkSimplex) * t),
                           (float*)(s.d_out + (sizeof(gkSimplex) + 4) * t),
                           s.with_f64 ? s.d_f64 : nullptr};
-    if ((e = ogjk::launch_posed(OGJK_MODE_GJK_EPA, a, (const float*)(s.d_in + 8 * t), o, total,
+    if ((e = ogjk::launch_posed(OGJK_MODE_GJK_EPA, a, (const float*)(s.d_in + 8 * t), false, o, total,
                                 s.d_ws, s.stream)) != hipSuccess)
       return e;
     if ((e = hipMemcpyAsync(s.h_out, s.d_out, kOutBytes * t, hipMemcpyDeviceToHost, s.stream)) !=

@@ -40,6 +40,7 @@ C_ABI_SYMBOLS = [
     "ogjk_collide_polytopes", "ogjk_collide_polytopes_indexed", "ogjk_collide_host",
     "ogjk_collide_host_indexed", "ogjk_cluster_slots", "ogjk_cluster_supers", "ogjk_build_clusters",
     "ogjk_graph_capture_scene", "ogjk_graph_launch", "ogjk_graph_destroy", "ogjk_collide_posed",
+    "ogjk_collide_posed_bodies",
     "ogjk_server_create", "ogjk_server_submit", "ogjk_server_wait", "ogjk_server_poll",
     "ogjk_server_info", "ogjk_server_destroy", "ogjk_broad_phase_workspace", "ogjk_broad_phase",
     "ogjk_expand_pairs_workspace", "ogjk_expand_pairs",
@@ -108,6 +109,8 @@ def lib():
             "ogjk_graph_launch": (C.c_int, [vp, vp]),
             "ogjk_collide_posed": (C.c_int, [C.c_int, C.POINTER(_Scene), vp, vp, i32, vp, vp, vp,
                                              vp, vp, C.c_size_t, vp]),
+            "ogjk_collide_posed_bodies": (C.c_int, [C.c_int, C.POINTER(_Scene), vp, vp, i32, vp,
+                                                    vp, vp, vp, vp, C.c_size_t, vp]),
             "ogjk_server_create": (C.c_int, [C.POINTER(_Scene), C.POINTER(_ServerConfig),
                                              C.POINTER(vp)]),
             "ogjk_server_submit": (C.c_int, [vp, vp, vp, i32, vp, vp, vp, vp, C.POINTER(i64)]),
@@ -291,6 +294,26 @@ class DeviceScene:
                                         out["distances"].data_ptr(), out["normals"].data_ptr(), f,
                                         ws.data_ptr(), need, self._stream(stream)))
         out["_workspace"] = ws  # alive until the caller drops the outputs
+        return out
+
+    def collide_posed_bodies(self, mode: int, pairs, body_poses, out=None,
+                             f64_record: bool = False, workspace=None, stream=None):
+        """Scene bodies under one pose per body (ogjk_collide_posed_bodies):
+        pairs (n, 2) int32 (a broad-phase list) and body_poses (n_bodies, 7)
+        float32 device tensors; asynchronous.  workspace: a uint8 device
+        tensor of at least ogjk_workspace_bytes(n) bytes, else allocated."""
+        n = int(pairs.shape[0])
+        out = out or self.outputs(n, f64_record)
+        need = int(lib().ogjk_workspace_bytes(n))
+        ws = workspace
+        if ws is None or ws.numel() < need:
+            ws = self.torch.empty(max(need, 1), dtype=self.torch.uint8, device=self.x.device)
+        f = out["f64"].data_ptr() if out.get("f64") is not None else None
+        _check(lib().ogjk_collide_posed_bodies(
+            mode, C.byref(self.scene), pairs.data_ptr(), body_poses.data_ptr(), n,
+            out["simplices"].data_ptr(), out["distances"].data_ptr(), out["normals"].data_ptr(), f,
+            ws.data_ptr(), ws.numel(), self._stream(stream)))
+        out["_workspace"] = ws
         return out
 
     def capture(self, mode: int, n_pairs: int, pairs=None, out=None, f64_record: bool = False):
@@ -528,3 +551,35 @@ def expand_pairs(body_pairs, sub_off, sub_cnt, max_sub_pairs: int | None = None,
         max_sub_pairs, C.byref(m), C.byref(found), ws.data_ptr(), ws.numel(),
         _torch_stream(stream)))
     return out[:m.value], found.value
+
+
+class SceneStep:
+    """Broad phase then narrow phase over a resident scene whose bodies move
+    by one pose each (the reference's sim_broad_phase + sim_step without the
+    integrator, visualization/integrate_final_gjk.cu:916-1035).  Body b's
+    bounding sphere is centred at its pose's translation with radius radii[b]
+    (which must bound |x| over the body's local vertices).  Buffers for
+    max_pairs pairs are allocated once."""
+
+    def __init__(self, scene: "DeviceScene", radii, grid: Grid, max_pairs: int | None = None,
+                 f64_record: bool = False):
+        import torch
+        self.torch, self.scene, self.grid = torch, scene, grid
+        nb = scene.n_bodies
+        dev = scene.x.device
+        self.spheres = torch.zeros((nb, 4), dtype=torch.float32, device=dev)
+        self.spheres[:, 3] = torch.as_tensor(np.asarray(radii, np.float32), device=dev)
+        self.bp = BroadPhase(nb, grid, max_pairs, device=dev)
+        self.out = scene.outputs(self.bp.max_pairs, f64_record)
+        self.workspace = torch.empty(max(int(lib().ogjk_workspace_bytes(self.bp.max_pairs)), 1),
+                                     dtype=torch.uint8, device=dev)
+
+    def __call__(self, body_poses, mode: int = GJK_EPA, stream=None):
+        """body_poses (n_bodies, 7) float32 device tensor.  Returns (pairs,
+        found, outputs): outputs hold the first len(pairs) results."""
+        self.spheres[:, :3].copy_(body_poses[:, 4:7])
+        pairs, found = self.bp(self.spheres, stream)
+        if len(pairs):
+            self.scene.collide_posed_bodies(mode, pairs, body_poses, out=self.out,
+                                            workspace=self.workspace, stream=stream)
+        return pairs, found, self.out

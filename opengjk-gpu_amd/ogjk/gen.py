@@ -120,3 +120,29 @@ def spheres(n: int, seed: int = 5, box: float = 10.0, rmin: float = 0.25, rmax: 
         c[m] *= rng.uniform(1.0, 2.0, size=(int(m.sum()), 1))
     r = rng.uniform(rmin, rmax, size=(n, 1))
     return np.concatenate([c, r], 1).astype(np.float32)
+
+
+def body_poses(n_bodies: int, seed: int = 6, tbox: float = 10.0):
+    """One pose per body: unit quaternion (w, x, y, z) = normalised 4 x N(0,1),
+    translation U[-tbox, tbox]^3.  (n_bodies, 7) float32."""
+    rng = np.random.default_rng(seed)
+    q = rng.normal(size=(n_bodies, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    t = rng.uniform(-tbox, tbox, size=(n_bodies, 3))
+    return np.concatenate([q, t], 1).astype(np.float32)
+
+
+def body_radii(coords, off, cnt):
+    """Bounding radius about the local origin of each body, max |x|_2 over its
+    vertices, rounded up to float32 and widened by 2^-20 relative so that it
+    bounds the FP64 world vertices of any pose (|R x| = |x| up to the
+    rounding of R from a float32 quaternion)."""
+    c = np.asarray(coords, np.float64).reshape(-1, 3)
+    r2 = (c * c).sum(1)
+    off = np.asarray(off, np.int64)
+    cnt = np.asarray(cnt, np.int64)
+    out = np.zeros(len(cnt), np.float64)
+    nz = cnt > 0
+    out[nz] = np.sqrt(np.maximum.reduceat(r2, off[nz]))
+    r = (out * (1.0 + 2.0 ** -20)).astype(np.float32)
+    return np.nextafter(r, np.float32(np.inf))
