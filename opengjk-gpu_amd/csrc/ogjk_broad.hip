@@ -3,10 +3,12 @@
 //
 // Same pair set and order as the reference's grid
 // (visualization/integrate_final_gjk.cu:467-570, 916-1002, 265-302), built the
-// other way round: instead of per-cell slot arrays filled by atomics, objects
-// are radix-sorted by cell (stable, so each cell lists its objects in
-// ascending index, one of the orders the reference's atomics can produce) and
-// a start table gives each cell's range.  Threads walk objects in cell order,
+// other way round: instead of fixed-size per-cell slot arrays filled by
+// atomics, objects are counting-sorted by cell into one array (a histogram by
+// atomics, one scan that is also the cell start table, placement, then each
+// object's rank among the lower indices of its cell, so every cell lists its
+// objects in ascending index, one of the orders the reference's atomics can
+// produce; no cell overflows).  Threads walk objects in cell order,
 // so the lanes of a wave query the same neighbour cells and their loads
 // coalesce or hit in cache; the three x-neighbours of a row are one contiguous
 // range when cells are not capped.
@@ -25,7 +27,7 @@ ogjk_status ogjk_fail(ogjk_status code, const char* fmt, ...);
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int64_t kMaxCells = int64_t(1) << 30;
+constexpr int64_t kMaxCells = (int64_t(1) << 30) - 1;  // cells + 1 scanned as int
 
 __device__ __forceinline__ int cell_axis(float p, float boundary, float cell, int g) {
   const float f = floorf((p + boundary) / cell);
@@ -41,66 +43,117 @@ __device__ __forceinline__ bool overlap(const float4 a, const float4 b) {
   return d2 < rs * rs;
 }
 
-__global__ __launch_bounds__(kBlock) void k_bp_keys(const float4* __restrict__ sph, int n,
+__global__ __launch_bounds__(kBlock) void k_bp_zero(int32_t* __restrict__ a, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    a[i] = 0;
+}
+
+// keys[i] = cell of object i; slot[i] = its arrival slot in the cell's count.
+__global__ __launch_bounds__(kBlock) void k_bp_hist(const float4* __restrict__ sph, int n,
                                                     float cell, float boundary, int g,
                                                     uint32_t* __restrict__ keys,
-                                                    int32_t* __restrict__ ids) {
+                                                    int32_t* __restrict__ slot,
+                                                    int32_t* __restrict__ count) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const float4 s = sph[i];
   const uint32_t cx = cell_axis(s.x, boundary, cell, g);
   const uint32_t cy = cell_axis(s.y, boundary, cell, g);
   const uint32_t cz = cell_axis(s.z, boundary, cell, g);
-  keys[i] = cx + (uint32_t)g * (cy + (uint32_t)g * cz);
-  ids[i] = i;
+  const uint32_t k = cx + (uint32_t)g * (cy + (uint32_t)g * cz);
+  keys[i] = k;
+  slot[i] = atomicAdd(count + k, 1);
 }
 
-// start[c] = first sorted position whose cell is >= c, for c in [0, cells].
-__global__ __launch_bounds__(kBlock) void k_bp_cell_start(const uint32_t* __restrict__ keys,
-                                                          int n, int64_t cells,
-                                                          int32_t* __restrict__ start) {
-  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (c > cells) return;
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if ((int64_t)keys[mid] < c) lo = mid + 1; else hi = mid;
-  }
-  start[c] = lo;
+__global__ __launch_bounds__(kBlock) void k_bp_place(const uint32_t* __restrict__ keys,
+                                                     const int32_t* __restrict__ slot,
+                                                     const int32_t* __restrict__ start, int n,
+                                                     int32_t* __restrict__ tmp) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) tmp[start[keys[i]] + slot[i]] = i;
 }
 
-__global__ __launch_bounds__(kBlock) void k_bp_gather(const float4* __restrict__ sph,
-                                                      const int32_t* __restrict__ ids, int n,
-                                                      float4* __restrict__ sorted) {
-  const int s = blockIdx.x * kBlock + threadIdx.x;
-  if (s < n) sorted[s] = sph[ids[s]];
+// An object in cell order: its sphere, index and cell, one 32-byte line.
+struct alignas(32) BpRec {
+  float4 s;
+  int32_t id;
+  uint32_t key;
+  int32_t pad[2];
+};
+
+// Object i goes to start[cell] + (number of lower indices in its cell): cells
+// list their objects in ascending index.  pos[i] = that position.
+__global__ __launch_bounds__(kBlock) void k_bp_rank(const float4* __restrict__ sph,
+                                                    const uint32_t* __restrict__ keys,
+                                                    const int32_t* __restrict__ start,
+                                                    const int32_t* __restrict__ tmp, int n,
+                                                    BpRec* __restrict__ rec,
+                                                    int32_t* __restrict__ pos) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = keys[i];
+  const int lo = start[k], hi = start[k + 1];
+  int r = 0;
+  for (int t = lo; t < hi; t++) r += tmp[t] < i;
+  BpRec e;
+  e.s = sph[i];
+  e.id = i;
+  e.key = k;
+  e.pad[0] = e.pad[1] = 0;
+  rec[lo + r] = e;
+  pos[i] = lo + r;
 }
 
-// One thread per object, in cell order.  Count pass: counts[o] = pairs (o, j),
-// j > o (counts[n] = 0 so the exclusive scan's last entry is the total).
-// Emit pass: writes them at offsets[o], in neighbour-cell order (z, y, x from
-// -1 to +1), ascending j within a cell; positions at or past max_pairs are
-// dropped.
-template <bool EMIT>
-__global__ __launch_bounds__(kBlock) void k_bp_pairs(
-    const float4* __restrict__ sorted, const int32_t* __restrict__ ids,
-    const uint32_t* __restrict__ keys, const int32_t* __restrict__ start, int n, int g, int cap,
-    int64_t* __restrict__ counts, const int64_t* __restrict__ offsets,
-    gkCollisionPair* __restrict__ pairs, int64_t max_pairs) {
-  const int s = blockIdx.x * kBlock + threadIdx.x;
-  if (s >= n) {
-    if (!EMIT && s == n) counts[n] = 0;
-    return;
-  }
-  const float4 p = sorted[s];
-  const int o = ids[s];
-  const uint32_t k = keys[s];
+// Partners of an object the count pass keeps for the emit pass.
+constexpr int kKeep = 8;
+
+// Calls f(j) for every j > o overlapping the object at sorted position s, in
+// neighbour-cell order (z, y, x from -1 to +1), ascending j within a cell.
+template <typename F>
+__device__ __forceinline__ void bp_walk(const BpRec* __restrict__ rec,
+                                        const int32_t* __restrict__ start, int s, int g, int cap,
+                                        F&& f) {
+  const BpRec me = rec[s];
+  const int o = me.id;
+  const uint32_t k = me.key;
   const int cx = (int)(k % (uint32_t)g);
   const int cy = (int)((k / (uint32_t)g) % (uint32_t)g);
   const int cz = (int)(k / ((uint32_t)g * (uint32_t)g));
   const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx < g - 1 ? cx + 1 : g - 1;
-  int64_t w = EMIT ? offsets[o] : 0;
-  int64_t cnt = 0;
+  if (cap == 0) {
+    // uncapped: the x-neighbours of each of the 9 rows are one contiguous
+    // range; all 18 bounds are loaded before any candidate (indices clamped
+    // into the grid, rows outside it emptied)
+    int lo[9], hi[9];
+#pragma unroll
+    for (int r = 0; r < 9; r++) {
+      const int nz = cz + r / 3 - 1, ny = cy + r % 3 - 1;
+      const bool in = nz >= 0 && nz < g && ny >= 0 && ny < g;
+      const int64_t row =
+          (int64_t)g * ((in ? ny : cy) + (int64_t)g * (in ? nz : cz));
+      lo[r] = start[row + x0];
+      hi[r] = in ? start[row + x1 + 1] : lo[r];
+    }
+#pragma unroll
+    for (int r = 0; r < 9; r++)
+      for (int t = lo[r]; t < hi[r]; t += 4) {
+        // four candidates in flight; indices past the row repeat its last
+        // entry and are skipped
+        float4 q[4];
+        int j[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int tu = t + u < hi[r] ? t + u : hi[r] - 1;
+          q[u] = rec[tu].s;
+          j[u] = rec[tu].id;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          if (t + u < hi[r] && j[u] > o && overlap(me.s, q[u])) f(j[u]);
+      }
+    return;
+  }
   for (int dz = -1; dz <= 1; dz++) {
     const int nz = cz + dz;
     if (nz < 0 || nz >= g) continue;
@@ -108,26 +161,82 @@ __global__ __launch_bounds__(kBlock) void k_bp_pairs(
       const int ny = cy + dy;
       if (ny < 0 || ny >= g) continue;
       const int64_t row = (int64_t)g * (ny + (int64_t)g * nz);
-      for (int nx = cap > 0 ? x0 : x1; nx <= x1; nx++) {
-        // uncapped: the row's x-neighbours are one contiguous range
-        int lo = start[row + (cap > 0 ? nx : x0)];
+      for (int nx = x0; nx <= x1; nx++) {
+        const int lo = start[row + nx];
         int hi = start[row + nx + 1];
-        if (cap > 0 && hi - lo > cap) hi = lo + cap;
+        if (hi - lo > cap) hi = lo + cap;
         for (int t = lo; t < hi; t++) {
-          const int j = ids[t];
-          if (j <= o) continue;
-          if (!overlap(p, sorted[t])) continue;
-          if (EMIT) {
-            if (w < max_pairs) pairs[w] = gkCollisionPair{o, j};
-            w++;
-          } else {
-            cnt++;
-          }
+          const float4 q = rec[t].s;
+          const int j = rec[t].id;
+          if (j > o && overlap(me.s, q)) f(j);
         }
       }
     }
   }
-  if (!EMIT) counts[o] = cnt;
+}
+
+// Workgroups go to the 8 XCDs round-robin; a grid of a multiple of 8 blocks
+// is remapped so that each XCD takes one contiguous eighth (a slab of cells
+// whose neighbour rows mostly stay in that XCD's L2).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block() {
+  const int per = gridDim.x / kXcds;
+  return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+unsigned xcd_blocks(int64_t threads) {
+  const int64_t b = (threads + kBlock - 1) / kBlock;
+  return (unsigned)((b + kXcds - 1) / kXcds * kXcds);
+}
+
+// Count pass, one thread per object in cell order: counts[o] = number of
+// pairs (o, j), j > o (counts[n] = 0, so the exclusive scan's last entry is
+// the total), and the first kKeep partners in keep[kKeep o ..].  Launched on
+// xcd_blocks(n + 1) blocks.
+__global__ __launch_bounds__(kBlock) void k_bp_count(const BpRec* __restrict__ rec,
+                                                     const int32_t* __restrict__ start, int n,
+                                                     int g, int cap,
+                                                     int64_t* __restrict__ counts,
+                                                     int32_t* __restrict__ keep) {
+  const int s = xcd_block() * kBlock + threadIdx.x;
+  if (s >= n) {
+    if (s == n) counts[n] = 0;
+    return;
+  }
+  const int o = rec[s].id;
+  int64_t cnt = 0;
+  int32_t* kp = keep + (int64_t)kKeep * o;
+  bp_walk(rec, start, s, g, cap, [&](int j) {
+    if (cnt < kKeep) kp[cnt] = j;
+    cnt++;
+  });
+  counts[o] = cnt;
+}
+
+// Emit pass, one thread per object in index order (the output is in index
+// order): copies the kept partners to offsets[o] or, for an object with more
+// than kKeep, walks its neighbours again; positions at or past max_pairs are
+// dropped.
+__global__ __launch_bounds__(kBlock) void k_bp_emit(const BpRec* __restrict__ rec,
+                                                    const int32_t* __restrict__ start,
+                                                    const int32_t* __restrict__ pos, int n, int g,
+                                                    int cap, const int64_t* __restrict__ counts,
+                                                    const int32_t* __restrict__ keep,
+                                                    const int64_t* __restrict__ offsets,
+                                                    gkCollisionPair* __restrict__ pairs,
+                                                    int64_t max_pairs) {
+  const int o = blockIdx.x * kBlock + threadIdx.x;
+  if (o >= n) return;
+  int64_t w = offsets[o];
+  const int64_t c = counts[o];
+  if (c <= kKeep) {
+    const int32_t* kp = keep + (int64_t)kKeep * o;
+    for (int q = 0; q < c && w < max_pairs; q++, w++) pairs[w] = gkCollisionPair{o, kp[q]};
+    return;
+  }
+  bp_walk(rec, start, pos[o], g, cap, [&](int j) {
+    if (w < max_pairs) pairs[w] = gkCollisionPair{o, j};
+    w++;
+  });
 }
 
 // Sub-mesh expansion: counts[p] = sub_cnt[a] * sub_cnt[b] (0 for an index out
@@ -168,38 +277,33 @@ __global__ __launch_bounds__(kBlock) void k_sub_expand(
 
 inline size_t up256(size_t b) { return (b + 255) & ~size_t(255); }
 
-int key_bits(int64_t cells) {
-  int b = 1;
-  while ((int64_t(1) << b) < cells) b++;
-  return b;
-}
-
 // Workspace of the broad phase for n objects over `cells` cells.
 struct BpLayout {
-  size_t keys_in, ids_in, keys_out, ids_out, sorted, counts, offsets, start, temp, total;
+  size_t keys, slot, count, start, tmp, rec, pos, keep, counts, offsets, temp, total;
   size_t temp_bytes;
 };
 
 hipError_t bp_layout(int n, int64_t cells, BpLayout* L) {
-  size_t sort_b = 0, scan_b = 0;
-  hipError_t e = hipcub::DeviceRadixSort::SortPairs(
-      nullptr, sort_b, (const uint32_t*)nullptr, (uint32_t*)nullptr, (const int32_t*)nullptr,
-      (int32_t*)nullptr, n, 0, key_bits(cells));
+  size_t cell_b = 0, scan_b = 0;
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, cell_b, (const int32_t*)nullptr,
+                                                  (int32_t*)nullptr, (int)(cells + 1));
   if (e != hipSuccess) return e;
   e = hipcub::DeviceScan::ExclusiveSum(nullptr, scan_b, (const int64_t*)nullptr,
                                        (int64_t*)nullptr, n + 1);
   if (e != hipSuccess) return e;
-  L->temp_bytes = sort_b > scan_b ? sort_b : scan_b;
+  L->temp_bytes = cell_b > scan_b ? cell_b : scan_b;
   size_t o = 0;
-  L->keys_in = o;  o += up256(sizeof(uint32_t) * (size_t)n);
-  L->ids_in = o;   o += up256(sizeof(int32_t) * (size_t)n);
-  L->keys_out = o; o += up256(sizeof(uint32_t) * (size_t)n);
-  L->ids_out = o;  o += up256(sizeof(int32_t) * (size_t)n);
-  L->sorted = o;   o += up256(sizeof(float4) * (size_t)n);
-  L->counts = o;   o += up256(sizeof(int64_t) * ((size_t)n + 1));
-  L->offsets = o;  o += up256(sizeof(int64_t) * ((size_t)n + 1));
-  L->start = o;    o += up256(sizeof(int32_t) * ((size_t)cells + 1));
-  L->temp = o;     o += up256(L->temp_bytes);
+  L->keys = o;    o += up256(sizeof(uint32_t) * (size_t)n);
+  L->slot = o;    o += up256(sizeof(int32_t) * (size_t)n);
+  L->count = o;   o += up256(sizeof(int32_t) * ((size_t)cells + 1));
+  L->start = o;   o += up256(sizeof(int32_t) * ((size_t)cells + 1));
+  L->tmp = o;     o += up256(sizeof(int32_t) * (size_t)n);
+  L->rec = o;     o += up256(sizeof(BpRec) * (size_t)n);
+  L->pos = o;     o += up256(sizeof(int32_t) * (size_t)n);
+  L->keep = o;    o += up256(sizeof(int32_t) * kKeep * (size_t)n);
+  L->counts = o;  o += up256(sizeof(int64_t) * ((size_t)n + 1));
+  L->offsets = o; o += up256(sizeof(int64_t) * ((size_t)n + 1));
+  L->temp = o;    o += up256(L->temp_bytes);
   L->total = o;
   return hipSuccess;
 }
@@ -241,7 +345,7 @@ ogjk_status check_grid(const ogjk_grid* g, int64_t* cells) {
   if (g->max_per_cell < 0) return ogjk_fail(OGJK_ERR_INVALID, "max_per_cell < 0");
   const int64_t c = (int64_t)g->grid_size * g->grid_size * g->grid_size;
   if (c > kMaxCells)
-    return ogjk_fail(OGJK_ERR_UNSUPPORTED, "grid of %lld cells exceeds 2^30", (long long)c);
+    return ogjk_fail(OGJK_ERR_UNSUPPORTED, "grid of %lld cells exceeds 2^30 - 1", (long long)c);
   *cells = c;
   return OGJK_OK;
 }
@@ -283,39 +387,44 @@ ogjk_status ogjk_broad_phase(const float* d_spheres, int32_t n_objects, const og
   if (!d_workspace || workspace_bytes < L.total)
     return ogjk_fail(OGJK_ERR_INVALID, "workspace of %zu bytes needed", L.total);
   char* ws = (char*)d_workspace;
-  auto* keys_in = (uint32_t*)(ws + L.keys_in);
-  auto* ids_in = (int32_t*)(ws + L.ids_in);
-  auto* keys = (uint32_t*)(ws + L.keys_out);
-  auto* ids = (int32_t*)(ws + L.ids_out);
-  auto* sorted = (float4*)(ws + L.sorted);
+  auto* keys = (uint32_t*)(ws + L.keys);
+  auto* slot = (int32_t*)(ws + L.slot);
+  auto* count = (int32_t*)(ws + L.count);
+  auto* start = (int32_t*)(ws + L.start);
+  auto* tmp = (int32_t*)(ws + L.tmp);
+  auto* rec = (BpRec*)(ws + L.rec);
+  auto* pos = (int32_t*)(ws + L.pos);
+  auto* keep = (int32_t*)(ws + L.keep);
   auto* counts = (int64_t*)(ws + L.counts);
   auto* offsets = (int64_t*)(ws + L.offsets);
-  auto* start = (int32_t*)(ws + L.start);
   void* temp = ws + L.temp;
   const hipStream_t st_ = (hipStream_t)stream;
   const int n = n_objects, g = grid->grid_size;
   const float4* sph = (const float4*)d_spheres;
 
-  hipLaunchKernelGGL(k_bp_keys, dim3(blocks_for(n)), dim3(kBlock), 0, st_, sph, n,
-                     grid->cell_size, grid->boundary, g, keys_in, ids_in);
-  BP_HIP(hipGetLastError(), "cell keys");
+  // counting sort by cell; start[] is the exclusive scan of the histogram
+  const int64_t zb = blocks_for(cells + 1);
+  hipLaunchKernelGGL(k_bp_zero, dim3((unsigned)(zb < 4096 ? zb : 4096)), dim3(kBlock), 0, st_,
+                     count, cells + 1);
+  hipLaunchKernelGGL(k_bp_hist, dim3(blocks_for(n)), dim3(kBlock), 0, st_, sph, n,
+                     grid->cell_size, grid->boundary, g, keys, slot, count);
+  BP_HIP(hipGetLastError(), "cell histogram");
   size_t tb = L.temp_bytes;
-  BP_HIP(hipcub::DeviceRadixSort::SortPairs(temp, tb, keys_in, keys, ids_in, ids, n, 0,
-                                            key_bits(cells), st_),
-         "cell sort");
-  hipLaunchKernelGGL(k_bp_cell_start, dim3(blocks_for(cells + 1)), dim3(kBlock), 0, st_, keys,
-                     n, cells, start);
-  hipLaunchKernelGGL(k_bp_gather, dim3(blocks_for(n)), dim3(kBlock), 0, st_, sph, ids, n,
-                     sorted);
-  hipLaunchKernelGGL(k_bp_pairs<false>, dim3(blocks_for((int64_t)n + 1)), dim3(kBlock), 0, st_,
-                     sorted, ids, keys, start, n, g, grid->max_per_cell, counts,
-                     (const int64_t*)nullptr, (gkCollisionPair*)nullptr, (int64_t)0);
+  BP_HIP(hipcub::DeviceScan::ExclusiveSum(temp, tb, count, start, (int)(cells + 1), st_),
+         "cell scan");
+  hipLaunchKernelGGL(k_bp_place, dim3(blocks_for(n)), dim3(kBlock), 0, st_, keys, slot, start,
+                     n, tmp);
+  hipLaunchKernelGGL(k_bp_rank, dim3(blocks_for(n)), dim3(kBlock), 0, st_, sph, keys, start, tmp,
+                     n, rec, pos);
+  BP_HIP(hipGetLastError(), "cell sort");
+  hipLaunchKernelGGL(k_bp_count, dim3(xcd_blocks((int64_t)n + 1)), dim3(kBlock), 0, st_, rec,
+                     start, n, g, grid->max_per_cell, counts, keep);
   BP_HIP(hipGetLastError(), "pair count");
   tb = L.temp_bytes;
   BP_HIP(hipcub::DeviceScan::ExclusiveSum(temp, tb, counts, offsets, n + 1, st_), "pair scan");
   if (max_pairs > 0) {
-    hipLaunchKernelGGL(k_bp_pairs<true>, dim3(blocks_for(n)), dim3(kBlock), 0, st_, sorted, ids,
-                       keys, start, n, g, grid->max_per_cell, counts, offsets, d_pairs,
+    hipLaunchKernelGGL(k_bp_emit, dim3(blocks_for(n)), dim3(kBlock), 0, st_, rec, start, pos, n,
+                       g, grid->max_per_cell, counts, keep, offsets, d_pairs,
                        (int64_t)max_pairs);
     BP_HIP(hipGetLastError(), "pair emit");
   }
