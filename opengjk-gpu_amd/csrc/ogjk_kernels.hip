@@ -15,6 +15,9 @@
 #ifndef OGJK_GJK_WAVES_WIDE
 #define OGJK_GJK_WAVES_WIDE 3
 #endif
+#ifndef OGJK_GJK_WAVES_POSED
+#define OGJK_GJK_WAVES_POSED 3
+#endif
 #ifndef OGJK_EPA_WAVES
 #define OGJK_EPA_WAVES 2
 #endif
@@ -92,7 +95,7 @@ struct PoseSrc {
   const float* poses;
   int32_t by_body;  // poses[7 b] per body b, else poses[14 i], [14 i + 7] per pair side
   using BodyT = PBody;
-  static constexpr int kGjkWaves = OGJK_GJK_WAVES_WIDE;
+  static constexpr int kGjkWaves = OGJK_GJK_WAVES_POSED;
   static constexpr bool kStage = false;
 
   OGJK_DEV int pair_size(int i) const {
