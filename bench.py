@@ -370,7 +370,8 @@ def scene_step(args) -> int:
     log-uniform ellipsoids in local frames), one pose per body (uniform unit
     quaternion, translation U[-40,40]^3), bounding spheres through the device
     broad phase (grid of at most 64^3 cells, cells one largest diameter
-    wide), then GJK+EPA on every pair found.  A step is both phases."""
+    wide), then GJK+EPA on every pair found, written as 40-byte contact
+    records.  A step is both phases."""
     import torch
     import oracle
     import ogjk
@@ -384,7 +385,7 @@ def scene_step(args) -> int:
     poses = gen.body_poses(nb, seed=7, tbox=tbox)
     grid = ogjk.Grid.for_spheres(float(radii.max()), tbox, 64)
     sc = ogjk.DeviceScene(b)
-    step = ogjk.SceneStep(sc, radii, grid, max_pairs=20 * nb, f64_record=False)
+    step = ogjk.SceneStep(sc, radii, grid, max_pairs=20 * nb, contacts=True)
     dp = torch.from_numpy(poses).cuda()
     pairs, found, out = step(dp)
     torch.cuda.synchronize()
@@ -393,11 +394,11 @@ def scene_step(args) -> int:
     wall, ms = _event_steps(torch, lambda: step(dp), args.steps, args.warmup)
     # narrow phase alone on the same pairs, for the kernel roofline
     ws = step.workspace
-    nwall, nms = _event_steps(torch, lambda: sc.collide_posed_bodies(3, pairs, dp, out=out,
-                                                                     workspace=ws),
-                              args.steps, args.warmup)
+    nwall, nms = _event_steps(torch, lambda: sc.collide_posed_bodies_contacts(
+        3, pairs, dp, out=out, workspace=ws), args.steps, args.warmup)
     cnt = b.cnt.astype(np.int64)
-    alg = float(12 * (cnt[got[:, 0]] + cnt[got[:, 1]]).sum() + n * (8 + 56 + 32 + 124))
+    # vertices once per pair side, pair 8 B, two poses 56 B, body table 32 B, contact record 40 B
+    alg = float(12 * (cnt[got[:, 0]] + cnt[got[:, 1]]).sum() + n * (8 + 56 + 32 + 40))
     achieved = alg / (np.mean(nms) * 1e-3) / 1e9
     line = {
         "metric": "scene step pairs/s (broad phase + posed GJK+EPA on the pairs found)",
@@ -413,8 +414,8 @@ def scene_step(args) -> int:
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("scene_step"),
                      "kernel": f"k_gjk + k_epa_list over the {n} pairs of a step",
                      "kernel_ms_mean": float(np.mean(nms)), "algorithmic_bytes_per_launch": alg,
-                     "algorithmic_bytes": "compulsory: vertices once + pair, poses, record, "
-                                          "results"},
+                     "algorithmic_bytes": "compulsory: vertices once + pair, poses, body "
+                                          "table, 40-byte contact record"},
     }
     if not args.no_cpu:
         cores = min(len(os.sched_getaffinity(0)), 16)

@@ -182,6 +182,35 @@ ogjk_status ogjk_collide_posed_bodies(int mode, const ogjk_scene* scene,
                                       float* d_distances, float* d_normals, double* d_f64,
                                       void* d_workspace, size_t workspace_bytes, void* stream);
 
+/* Compact contact record, 40 B per pair: what a contact consumer reads.  The
+ * reference's collision_response_kernel reads distances[p], epa_normals[3p..]
+ * and simplices[p].witnesses (visualization/integrate_final_gjk.cu:572-689),
+ * 40 of the 124 bytes the gkSimplex outputs hold.  Values are those of the
+ * gkSimplex/distance/normal outputs of the same launch, bit for bit; the
+ * normal is zero in GJK-only mode, and a pair with an invalid body has every
+ * field NaN. */
+typedef struct ogjk_contact {
+  float distance;      /* > 0: separation; <= 0: -penetration depth */
+  float normal[3];
+  float witness[2][3]; /* on A, on B */
+} ogjk_contact;
+OGJK_STATIC_ASSERT(sizeof(ogjk_contact) == 40, "ogjk_contact layout");
+OGJK_STATIC_ASSERT(offsetof(ogjk_contact, witness) == 16, "ogjk_contact.witness");
+
+/* As ogjk_collide_scene_ws and ogjk_collide_posed_bodies, writing contact
+ * records instead of gkSimplex, distance and normal arrays (mode 1 or 3;
+ * d_f64 optional).  No counterpart in the reference: its consumer reads the
+ * three arrays above. */
+ogjk_status ogjk_collide_scene_contacts(int mode, const ogjk_scene* scene, const int32_t* d_pairs,
+                                        int32_t n_pairs, ogjk_contact* d_contacts, double* d_f64,
+                                        void* d_workspace, size_t workspace_bytes, void* stream);
+ogjk_status ogjk_collide_posed_bodies_contacts(int mode, const ogjk_scene* scene,
+                                               const gkCollisionPair* d_pairs,
+                                               const float* d_body_poses, int32_t n_pairs,
+                                               ogjk_contact* d_contacts, double* d_f64,
+                                               void* d_workspace, size_t workspace_bytes,
+                                               void* stream);
+
 /* Serving loop over a resident scene (no counterpart in the reference, whose
  * calls are synchronous and allocate per call, GJK/gpu/openGJK.cu:2791-2883).
  * Requests of posed pairs (layout of ogjk_collide_posed, host memory) are

@@ -316,6 +316,52 @@ ogjk_status ogjk_collide_scene_ws(int mode, const ogjk_scene* scene, const int32
   return OGJK_OK;
 }
 
+static ogjk_status check_contacts(int mode, int32_t n, ogjk_contact* contacts, void* ws,
+                                  size_t ws_bytes) {
+  if (mode != OGJK_MODE_GJK && mode != OGJK_MODE_GJK_EPA)
+    return fail(OGJK_ERR_INVALID, "contact records need mode 1 or 3, got %d", mode);
+  if (n < 0) return fail(OGJK_ERR_INVALID, "n = %d < 0", n);
+  if (n > 0 && !contacts) return fail(OGJK_ERR_INVALID, "NULL contacts");
+  if (n > 0 && (!ws || ws_bytes < ogjk::workspace_bytes(n)))
+    return fail(OGJK_ERR_INVALID, "workspace of %zu bytes needed", ogjk::workspace_bytes(n));
+  return OGJK_OK;
+}
+
+ogjk_status ogjk_collide_scene_contacts(int mode, const ogjk_scene* scene, const int32_t* d_pairs,
+                                        int32_t n_pairs, ogjk_contact* d_contacts, double* d_f64,
+                                        void* d_workspace, size_t workspace_bytes, void* stream) {
+  const ogjk_status st = check_contacts(mode, n_pairs, d_contacts, d_workspace, workspace_bytes);
+  if (st != OGJK_OK || n_pairs == 0) return st;
+  ogjk::SceneArgs sa;
+  const ogjk_status s2 = scene_args(scene, d_pairs, n_pairs, &sa);
+  if (s2 != OGJK_OK) return s2;
+  ogjk::OutArgs oa{nullptr, nullptr, nullptr, d_f64};
+  oa.contacts = (float*)d_contacts;
+  OGJK_HIP(ogjk::launch_scene(mode, sa, oa, n_pairs, d_workspace, (hipStream_t)stream),
+           "kernel launch");
+  return OGJK_OK;
+}
+
+ogjk_status ogjk_collide_posed_bodies_contacts(int mode, const ogjk_scene* scene,
+                                               const gkCollisionPair* d_pairs,
+                                               const float* d_body_poses, int32_t n_pairs,
+                                               ogjk_contact* d_contacts, double* d_f64,
+                                               void* d_workspace, size_t workspace_bytes,
+                                               void* stream) {
+  const ogjk_status st = check_contacts(mode, n_pairs, d_contacts, d_workspace, workspace_bytes);
+  if (st != OGJK_OK || n_pairs == 0) return st;
+  if (!d_pairs || !d_body_poses) return fail(OGJK_ERR_INVALID, "posed pairs need pairs and poses");
+  ogjk::SceneArgs sa;
+  const ogjk_status s2 = scene_args(scene, (const int32_t*)d_pairs, n_pairs, &sa);
+  if (s2 != OGJK_OK) return s2;
+  ogjk::OutArgs oa{nullptr, nullptr, nullptr, d_f64};
+  oa.contacts = (float*)d_contacts;
+  OGJK_HIP(ogjk::launch_posed(mode, sa, d_body_poses, true, oa, n_pairs, d_workspace,
+                              (hipStream_t)stream),
+           "kernel launch");
+  return OGJK_OK;
+}
+
 ogjk_status ogjk_collide_posed(int mode, const ogjk_scene* scene, const int32_t* d_pairs,
                                const float* d_poses, int32_t n_pairs, gkSimplex* d_simplices,
                                float* d_distances, float* d_normals, double* d_f64,

@@ -39,11 +39,15 @@ struct PolyArgs {                // gkPolytope descriptors, AoS coordinates
 };
 
 struct OutArgs {
-  gkSimplex* simplices;  // GJK: out; EPA: in/out
-  float* distances;      // GJK: out; EPA: in/out
+  gkSimplex* simplices;  // GJK: out; EPA: in/out; null with contacts
+  float* distances;      // GJK: out; EPA: in/out; null with contacts
   float* normals;        // EPA: out (3 per pair), may be null
   double* f64;           // optional FP64 verification record, may be null
+  float* contacts = nullptr;  // optional compact record, OGJK_CONTACT_FLOATS per pair
 };
+
+// Floats per pair in the compact contact record (ogjk_contact).
+constexpr int OGJK_CONTACT_FLOATS = 10;
 
 // Doubles per pair in the optional FP64 verification record.
 constexpr int OGJK_F64_REC = 24;

@@ -168,8 +168,13 @@ struct PolySrc {
 
 OGJK_DEV void store_invalid(const OutArgs& o, int i, bool epa) {
   const double nan = __builtin_nan("");
-  o.distances[i] = (float)nan;
-  o.simplices[i].nvrtx = 0;
+  if (o.contacts) {
+    float* c = o.contacts + (int64_t)i * OGJK_CONTACT_FLOATS;
+#pragma unroll
+    for (int k = 0; k < OGJK_CONTACT_FLOATS; k++) c[k] = (float)nan;
+  }
+  if (o.distances) o.distances[i] = (float)nan;
+  if (o.simplices) o.simplices[i].nvrtx = 0;
   if (epa && o.normals) {
     o.normals[3 * i + 0] = (float)nan;
     o.normals[3 * i + 1] = (float)nan;
@@ -182,23 +187,34 @@ OGJK_DEV void store_invalid(const OutArgs& o, int i, bool epa) {
 
 OGJK_DEV void store(const OutArgs& o, int i, const Simplex& s, const Wit& w, double dist, D3 n,
                     bool epa) {
-  gkSimplex* r = o.simplices + i;
-  r->nvrtx = s.n;
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    r->vrtx[k][0] = (float)s.p[k].x;
-    r->vrtx[k][1] = (float)s.p[k].y;
-    r->vrtx[k][2] = (float)s.p[k].z;
-    r->vrtx_idx[k][0] = s.ia[k];
-    r->vrtx_idx[k][1] = s.ib[k];
+  if (o.contacts) {
+    // distance, normal (zero without EPA), witness on A, witness on B
+    float2* c = (float2*)(o.contacts + (int64_t)i * OGJK_CONTACT_FLOATS);
+    c[0] = float2{(float)dist, epa ? (float)n.x : 0.0f};
+    c[1] = float2{epa ? (float)n.y : 0.0f, epa ? (float)n.z : 0.0f};
+    c[2] = float2{(float)w.w0.x, (float)w.w0.y};
+    c[3] = float2{(float)w.w0.z, (float)w.w1.x};
+    c[4] = float2{(float)w.w1.y, (float)w.w1.z};
   }
-  r->witnesses[0][0] = (float)w.w0.x;
-  r->witnesses[0][1] = (float)w.w0.y;
-  r->witnesses[0][2] = (float)w.w0.z;
-  r->witnesses[1][0] = (float)w.w1.x;
-  r->witnesses[1][1] = (float)w.w1.y;
-  r->witnesses[1][2] = (float)w.w1.z;
-  o.distances[i] = (float)dist;
+  if (o.simplices) {
+    gkSimplex* r = o.simplices + i;
+    r->nvrtx = s.n;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      r->vrtx[k][0] = (float)s.p[k].x;
+      r->vrtx[k][1] = (float)s.p[k].y;
+      r->vrtx[k][2] = (float)s.p[k].z;
+      r->vrtx_idx[k][0] = s.ia[k];
+      r->vrtx_idx[k][1] = s.ib[k];
+    }
+    r->witnesses[0][0] = (float)w.w0.x;
+    r->witnesses[0][1] = (float)w.w0.y;
+    r->witnesses[0][2] = (float)w.w0.z;
+    r->witnesses[1][0] = (float)w.w1.x;
+    r->witnesses[1][1] = (float)w.w1.y;
+    r->witnesses[1][2] = (float)w.w1.z;
+  }
+  if (o.distances) o.distances[i] = (float)dist;
   if (epa && o.normals) {
     o.normals[3 * i + 0] = (float)n.x;
     o.normals[3 * i + 1] = (float)n.y;

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libogjk.so")
 
 GJK, EPA, GJK_EPA = 1, 2, 3
 F64_RECORD = 24
+CONTACT_FLOATS = 10  # ogjk_contact: distance, normal[3], witness[2][3]
 
 SIMPLEX_DTYPE = np.dtype({
     "names": ["nvrtx", "vrtx", "vrtx_idx", "witnesses"],
@@ -43,7 +44,8 @@ C_ABI_SYMBOLS = [
     "ogjk_collide_posed_bodies",
     "ogjk_server_create", "ogjk_server_submit", "ogjk_server_wait", "ogjk_server_poll",
     "ogjk_server_info", "ogjk_server_destroy", "ogjk_broad_phase_workspace", "ogjk_broad_phase",
-    "ogjk_expand_pairs_workspace", "ogjk_expand_pairs",
+    "ogjk_expand_pairs_workspace", "ogjk_expand_pairs", "ogjk_collide_scene_contacts",
+    "ogjk_collide_posed_bodies_contacts",
 ]
 
 
@@ -127,6 +129,10 @@ def lib():
             "ogjk_collide_polytopes_indexed": (C.c_int, [C.c_int, vp, i32, vp, i32, vp, vp, vp, vp]),
             "ogjk_collide_host": (C.c_int, [C.c_int, i32, vp, vp, vp, vp, vp]),
             "ogjk_collide_host_indexed": (C.c_int, [C.c_int, i32, i32, vp, vp, vp, vp, vp]),
+            "ogjk_collide_scene_contacts": (C.c_int, [C.c_int, C.POINTER(_Scene), vp, i32, vp,
+                                                      vp, vp, C.c_size_t, vp]),
+            "ogjk_collide_posed_bodies_contacts": (C.c_int, [C.c_int, C.POINTER(_Scene), vp, vp,
+                                                             i32, vp, vp, vp, C.c_size_t, vp]),
             "ogjk_broad_phase_workspace": (C.c_int, [i32, C.POINTER(_Grid),
                                                      C.POINTER(C.c_size_t)]),
             "ogjk_broad_phase": (C.c_int, [vp, i32, C.POINTER(_Grid), vp, i32, C.POINTER(i32),
@@ -313,6 +319,46 @@ class DeviceScene:
             mode, C.byref(self.scene), pairs.data_ptr(), body_poses.data_ptr(), n,
             out["simplices"].data_ptr(), out["distances"].data_ptr(), out["normals"].data_ptr(), f,
             ws.data_ptr(), ws.numel(), self._stream(stream)))
+        out["_workspace"] = ws
+        return out
+
+    def contacts(self, n: int, f64: bool = False):
+        """Output buffers for the contact-record entry points."""
+        t = self.torch
+        dev = self.x.device
+        return {"contacts": t.empty((max(n, 1), CONTACT_FLOATS), dtype=t.float32, device=dev),
+                "f64": t.empty((max(n, 1), F64_RECORD), dtype=t.float64, device=dev)
+                if f64 else None}
+
+    def _ws(self, n, workspace):
+        need = int(lib().ogjk_workspace_bytes(n))
+        if workspace is None or workspace.numel() < need:
+            workspace = self.torch.empty(max(need, 1), dtype=self.torch.uint8, device=self.x.device)
+        return workspace
+
+    def collide_contacts(self, mode: int, n_pairs: int, pairs=None, out=None,
+                         f64_record: bool = False, workspace=None, stream=None):
+        """ogjk_collide_scene_contacts: pairs (n, 2) int32 device tensor or
+        None for (2i, 2i+1); out["contacts"] is (n, 10) float32."""
+        out = out or self.contacts(n_pairs, f64_record)
+        ws = self._ws(n_pairs, workspace)
+        f = out["f64"].data_ptr() if out.get("f64") is not None else None
+        _check(lib().ogjk_collide_scene_contacts(
+            mode, C.byref(self.scene), None if pairs is None else pairs.data_ptr(), n_pairs,
+            out["contacts"].data_ptr(), f, ws.data_ptr(), ws.numel(), self._stream(stream)))
+        out["_workspace"] = ws
+        return out
+
+    def collide_posed_bodies_contacts(self, mode: int, pairs, body_poses, out=None,
+                                      f64_record: bool = False, workspace=None, stream=None):
+        """ogjk_collide_posed_bodies_contacts (see collide_posed_bodies)."""
+        n = int(pairs.shape[0])
+        out = out or self.contacts(n, f64_record)
+        ws = self._ws(n, workspace)
+        f = out["f64"].data_ptr() if out.get("f64") is not None else None
+        _check(lib().ogjk_collide_posed_bodies_contacts(
+            mode, C.byref(self.scene), pairs.data_ptr(), body_poses.data_ptr(), n,
+            out["contacts"].data_ptr(), f, ws.data_ptr(), ws.numel(), self._stream(stream)))
         out["_workspace"] = ws
         return out
 
@@ -559,10 +605,11 @@ class SceneStep:
     integrator, visualization/integrate_final_gjk.cu:916-1035).  Body b's
     bounding sphere is centred at its pose's translation with radius radii[b]
     (which must bound |x| over the body's local vertices).  Buffers for
-    max_pairs pairs are allocated once."""
+    max_pairs pairs are allocated once; contacts=True writes 40-byte contact
+    records (out["contacts"]) instead of the gkSimplex outputs."""
 
     def __init__(self, scene: "DeviceScene", radii, grid: Grid, max_pairs: int | None = None,
-                 f64_record: bool = False):
+                 f64_record: bool = False, contacts: bool = False):
         import torch
         self.torch, self.scene, self.grid = torch, scene, grid
         nb = scene.n_bodies
@@ -570,7 +617,9 @@ class SceneStep:
         self.spheres = torch.zeros((nb, 4), dtype=torch.float32, device=dev)
         self.spheres[:, 3] = torch.as_tensor(np.asarray(radii, np.float32), device=dev)
         self.bp = BroadPhase(nb, grid, max_pairs, device=dev)
-        self.out = scene.outputs(self.bp.max_pairs, f64_record)
+        self.use_contacts = contacts
+        self.out = (scene.contacts(self.bp.max_pairs, f64_record) if contacts
+                    else scene.outputs(self.bp.max_pairs, f64_record))
         self.workspace = torch.empty(max(int(lib().ogjk_workspace_bytes(self.bp.max_pairs)), 1),
                                      dtype=torch.uint8, device=dev)
 
@@ -580,6 +629,7 @@ class SceneStep:
         self.spheres[:, :3].copy_(body_poses[:, 4:7])
         pairs, found = self.bp(self.spheres, stream)
         if len(pairs):
-            self.scene.collide_posed_bodies(mode, pairs, body_poses, out=self.out,
-                                            workspace=self.workspace, stream=stream)
+            run = (self.scene.collide_posed_bodies_contacts if self.use_contacts
+                   else self.scene.collide_posed_bodies)
+            run(mode, pairs, body_poses, out=self.out, workspace=self.workspace, stream=stream)
         return pairs, found, self.out

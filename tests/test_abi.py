@@ -72,3 +72,33 @@ def test_invalid_arguments_fail_cleanly():
     assert lib.ogjk_set_config(24, 32, 0) == 1
     assert lib.ogjk_set_config(16, 8, 0) == 1
     assert lib.ogjk_set_config(16, 32, 0) == 0
+
+
+def test_broad_phase_and_contact_arguments_fail_cleanly():
+    """Argument errors of the broad-phase and contact-record entry points are
+    reported before any device work (no GPU needed)."""
+    import ctypes as C
+    lib = ogjk.lib()
+    g = ogjk._Grid(0.0, 10.0, 8, 0)
+    m, f = C.c_int32(), C.c_int64()
+    assert lib.ogjk_broad_phase(None, 10, C.byref(g), None, 0, C.byref(m), C.byref(f), None, 0,
+                                None) == 1
+    assert b"cell_size" in lib.ogjk_last_error()
+    g = ogjk._Grid(1.0, 10.0, 2000, 0)
+    assert lib.ogjk_broad_phase(None, 10, C.byref(g), None, 0, C.byref(m), C.byref(f), None, 0,
+                                None) == 4  # 8e9 cells: unsupported
+    g = ogjk._Grid(1.0, 10.0, 8, -1)
+    assert lib.ogjk_broad_phase(None, 10, C.byref(g), None, 0, C.byref(m), C.byref(f), None, 0,
+                                None) == 1
+    g = ogjk._Grid(1.0, 10.0, 8, 0)
+    assert lib.ogjk_broad_phase(None, 0, C.byref(g), None, 0, C.byref(m), C.byref(f), None, 0,
+                                None) == 0 and m.value == 0 and f.value == 0  # no objects
+    assert lib.ogjk_expand_pairs(None, 0, None, None, 0, None, 0, C.byref(m), C.byref(f), None,
+                                 0, None) == 0
+    assert lib.ogjk_expand_pairs(None, -1, None, None, 0, None, 0, C.byref(m), C.byref(f), None,
+                                 0, None) == 1
+    # contact records: EPA-only mode (needs simplex input) is refused
+    assert lib.ogjk_collide_scene_contacts(2, None, None, 5, None, None, None, 0, None) == 1
+    assert b"mode" in lib.ogjk_last_error()
+    assert lib.ogjk_collide_scene_contacts(3, None, None, 5, None, None, None, 0, None) == 1
+    assert lib.ogjk_collide_scene_contacts(3, None, None, 0, None, None, None, 0, None) == 0

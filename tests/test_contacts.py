@@ -1,0 +1,88 @@
+"""Contact records (ogjk_collide_scene_contacts, ogjk_collide_posed_bodies_contacts):
+each 40-byte record must equal the gkSimplex witnesses, distance and normal
+that the ABI outputs of the same launch hold, bit for bit."""
+import numpy as np
+import pytest
+
+import oracle
+import ogjk
+from ogjk import gen
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def as_records(out):
+    simp = ogjk.simplices_from_device(out["simplices"])
+    d = out["distances"].cpu().numpy()
+    nrm = out["normals"].cpu().numpy().reshape(-1, 3)
+    w = simp["witnesses"].reshape(-1, 6)
+    return np.concatenate([d[:, None], nrm, w], 1).astype(np.float32)
+
+
+def same_bits(a, b):
+    return np.array_equal(np.ascontiguousarray(a).view(np.uint32),
+                          np.ascontiguousarray(b).view(np.uint32))
+
+
+@pytest.mark.parametrize("mode", [oracle.GJK, oracle.GJK_EPA])
+def test_scene_contacts_equal_abi_outputs(torch_cuda, mode):
+    torch = torch_cuda
+    b = gen.make(gen.SPHERE, 8192, seed=61, nverts=64, box=1.3)
+    sc = ogjk.DeviceScene(b)
+    n = b.n_pairs
+    out = sc.collide(mode, n, f64_record=True)
+    con = sc.collide_contacts(mode, n, f64_record=True)
+    torch.cuda.synchronize()
+    ref = as_records(out)
+    got = con["contacts"][:n].cpu().numpy()
+    if mode == oracle.GJK:
+        ref[:, 1:4] = 0.0  # no normal without EPA
+    print(f"\n[contacts mode {mode}] {n} pairs, {(ref[:, 0] <= 0).mean():.2f} penetrating",
+          flush=True)
+    assert same_bits(got, ref)
+    assert np.array_equal(con["f64"][:n].cpu().numpy().view(np.uint64),
+                          out["f64"][:n].cpu().numpy().view(np.uint64))
+
+
+def test_scene_contacts_invalid_pairs_are_nan(torch_cuda):
+    torch = torch_cuda
+    b = gen.make(gen.SPHERE, 64, seed=62, nverts=32, box=1.3)
+    sc = ogjk.DeviceScene(b)
+    pairs = np.stack([np.arange(0, 128, 2), np.arange(1, 128, 2)], 1).astype(np.int32)
+    pairs[5, 1] = 10_000
+    pairs[9, 0] = -1
+    con = sc.collide_contacts(oracle.GJK_EPA, len(pairs), pairs=torch.from_numpy(pairs).cuda())
+    torch.cuda.synchronize()
+    got = con["contacts"][:len(pairs)].cpu().numpy()
+    assert np.isnan(got[[5, 9]]).all()
+    ok = np.ones(len(pairs), bool)
+    ok[[5, 9]] = False
+    assert np.isfinite(got[ok]).all()
+
+
+def test_scene_step_contacts_equal_abi_outputs(torch_cuda):
+    torch = torch_cuda
+    b = gen.make(gen.ELLIPSOID, 1000, seed=63, box=0.0, nmin=8, nmax=512)
+    nb = len(b.cnt)
+    radii = gen.body_radii(b.coords, b.off, b.cnt)
+    poses = torch.from_numpy(gen.body_poses(nb, seed=64, tbox=12.0)).cuda()
+    grid = ogjk.Grid.for_spheres(float(radii.max()), 12.0, 30)
+    sc = ogjk.DeviceScene(b)
+    full = ogjk.SceneStep(sc, radii, grid, max_pairs=20 * nb)
+    comp = ogjk.SceneStep(sc, radii, grid, max_pairs=20 * nb, contacts=True)
+    pairs, found, out = full(poses)
+    pairs2, found2, con = comp(poses)
+    torch.cuda.synchronize()
+    n = len(pairs)
+    print(f"\n[scene step contacts] {n} pairs", flush=True)
+    assert found == found2 and torch.equal(pairs, pairs2) and n > nb // 2
+    ref = as_records({k: v[:n] for k, v in out.items() if k != "_workspace" and v is not None})
+    assert same_bits(con["contacts"][:n].cpu().numpy(), ref)
