@@ -285,6 +285,42 @@ ogjk_status ogjk_expand_pairs(const gkCollisionPair* d_body_pairs, int32_t n_bod
                               int32_t max_sub_pairs, int32_t* n_sub_pairs, int64_t* n_found,
                               void* d_workspace, size_t workspace_bytes, void* stream);
 
+/* ---------------------------------------------------------- mesh ingestion */
+
+/* Convex hulls of n_clouds point clouds at once (cloud m: d_point_cnt[m]
+ * points at d_points + 3 * d_point_off[m], AoS float32).  Faces (vertex-index
+ * triples, local to the cloud) go to d_faces + 3 * d_face_off[m], at most
+ * d_face_off[m + 1] - d_face_off[m] of them; d_face_off has n_clouds + 1
+ * entries and d_face_off[n_clouds] == total_faces.  d_nfaces[m] = face count,
+ * 0 when there is no hull (fewer than 4 points, collinear or coplanar), -1
+ * when the capacity was too small, -2 for a non-finite coordinate.  The face
+ * list is the reference's, in its order, including its behaviour on dense
+ * near-coplanar input (more than 2n - 4 faces can result: give about 4n).
+ * Workspace: 12 * total_faces bytes.  Asynchronous.  Replaces build_hull
+ * (visualization/rendering/mesh_builder.cpp:279-360), run once per cloud on
+ * the host there. */
+ogjk_status ogjk_build_hulls(const float* d_points, const int64_t* d_point_off,
+                             const int32_t* d_point_cnt, int32_t n_clouds,
+                             const int64_t* d_face_off, int64_t total_faces, int32_t* d_faces,
+                             int32_t* d_nfaces, void* d_workspace, size_t workspace_bytes,
+                             void* stream);
+
+/* Per cloud, from its hull: d_moments[7m..7m+6] = volume, first moments
+ * (x, y, z, times 4 volume), second moments (x^2, y^2, z^2) in FP64, summed in
+ * face order; d_com[3m..] = the centre of mass center_hull_verts shifts by (0
+ * below volume 1e-10); d_inertia_k[3m..] = compute_hull_inertia_k's factors.
+ * Replaces hull_volume_moments, center_hull_verts' shift and
+ * compute_hull_inertia_k (visualization/rendering/mesh_builder.cpp:433-481). */
+ogjk_status ogjk_hull_moments(const float* d_points, const int64_t* d_point_off,
+                              const int32_t* d_faces, const int64_t* d_face_off,
+                              const int32_t* d_nfaces, int32_t n_clouds, double* d_moments,
+                              float* d_com, float* d_inertia_k, void* stream);
+
+/* d_points[i] -= d_com[d_point_cloud[i]] for n_points points (FP32), the
+ * reference's centring of a cloud (mesh_builder.cpp:464-469). */
+ogjk_status ogjk_center_points(float* d_points, const int32_t* d_point_cloud, int64_t n_points,
+                               const float* d_com, void* stream);
+
 /* Slots a cluster build needs for a scene of n_verts vertices in n_bodies
  * bodies (padding to 32 per body included). */
 int64_t ogjk_cluster_slots(int64_t n_verts, int32_t n_bodies);

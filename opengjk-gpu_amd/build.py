@@ -32,7 +32,7 @@ DEVICE_FLAGS = [
 ]
 
 LIB_SOURCES = ["ogjk_kernels.hip", "ogjk_cluster.hip", "ogjk_api.hip", "ogjk_serve.hip",
-               "ogjk_broad.hip", "ogjk_dropin.hip"]
+               "ogjk_broad.hip", "ogjk_hull.hip", "ogjk_dropin.hip"]
 
 
 def _run(cmd, cwd=None):

@@ -45,7 +45,8 @@ C_ABI_SYMBOLS = [
     "ogjk_server_create", "ogjk_server_submit", "ogjk_server_wait", "ogjk_server_poll",
     "ogjk_server_info", "ogjk_server_destroy", "ogjk_broad_phase_workspace", "ogjk_broad_phase",
     "ogjk_expand_pairs_workspace", "ogjk_expand_pairs", "ogjk_collide_scene_contacts",
-    "ogjk_collide_posed_bodies_contacts",
+    "ogjk_collide_posed_bodies_contacts", "ogjk_build_hulls", "ogjk_hull_moments",
+    "ogjk_center_points",
 ]
 
 
@@ -133,6 +134,10 @@ def lib():
                                                       vp, vp, C.c_size_t, vp]),
             "ogjk_collide_posed_bodies_contacts": (C.c_int, [C.c_int, C.POINTER(_Scene), vp, vp,
                                                              i32, vp, vp, vp, C.c_size_t, vp]),
+            "ogjk_build_hulls": (C.c_int, [vp, vp, vp, i32, vp, i64, vp, vp, vp, C.c_size_t,
+                                           vp]),
+            "ogjk_hull_moments": (C.c_int, [vp, vp, vp, vp, vp, i32, vp, vp, vp, vp]),
+            "ogjk_center_points": (C.c_int, [vp, vp, i64, vp, vp]),
             "ogjk_broad_phase_workspace": (C.c_int, [i32, C.POINTER(_Grid),
                                                      C.POINTER(C.c_size_t)]),
             "ogjk_broad_phase": (C.c_int, [vp, i32, C.POINTER(_Grid), vp, i32, C.POINTER(i32),
@@ -633,3 +638,86 @@ class SceneStep:
                    else self.scene.collide_posed_bodies)
             run(mode, pairs, body_poses, out=self.out, workspace=self.workspace, stream=stream)
         return pairs, found, self.out
+
+
+# ---------------------------------------------------------------- mesh ingestion
+
+def build_hulls(points, off, cnt, face_cap=None, stream=None):
+    """Convex hulls of many clouds on the device (ogjk_build_hulls).  points
+    (N, 3) float32, off int64 and cnt int32 per cloud, all device tensors;
+    face_cap: faces per cloud (device int64 tensor), default 4 n + 16.
+    Returns (faces (F, 3) int32, face_off (n + 1) int64, nfaces int32): cloud
+    m's faces are faces[face_off[m]:face_off[m] + nfaces[m]] (-1: capacity too
+    small, -2: non-finite input, 0: no hull)."""
+    import torch
+    n = int(cnt.numel())
+    dev = points.device
+    cap = (4 * cnt.long() + 16) if face_cap is None else face_cap.long()
+    face_off = torch.zeros(n + 1, dtype=torch.int64, device=dev)
+    if n:
+        face_off[1:] = torch.cumsum(cap, 0)
+    total = int(face_off[-1].item()) if n else 0
+    faces = torch.empty((max(total, 1), 3), dtype=torch.int32, device=dev)
+    nfaces = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+    ws = torch.empty(max(12 * total, 1), dtype=torch.uint8, device=dev)
+    _check(lib().ogjk_build_hulls(points.data_ptr(), off.data_ptr(), cnt.data_ptr(), n,
+                                  face_off.data_ptr(), total, faces.data_ptr(), nfaces.data_ptr(),
+                                  ws.data_ptr(), ws.numel(), _torch_stream(stream)))
+    return faces, face_off, nfaces[:n]
+
+
+def build_hulls_fit(points, off, cnt, stream=None):
+    """build_hulls at 4 n + 16 faces per cloud, then once more at 16 n + 64 for
+    the clouds that overflowed.  The reference's algorithm can grow its face
+    list without bound on dense, nearly coplanar surface points (its 1e-7
+    visibility threshold); a cloud still over 16 n + 64 keeps nfaces = -1, as
+    the oracle reports it at that capacity, and its build time stays bounded."""
+    import torch
+    faces, face_off, nfaces = build_hulls(points, off, cnt, 4 * cnt.long() + 16, stream)
+    over = nfaces == -1
+    if bool(over.any()):
+        cap = torch.where(over, 16 * cnt.long() + 64, 4 * cnt.long() + 16)
+        faces, face_off, nfaces = build_hulls(points, off, cnt, cap, stream)
+    return faces, face_off, nfaces
+
+
+def hull_moments(points, off, faces, face_off, nfaces, stream=None):
+    """Per cloud: FP64 moments (n, 7), centre of mass (n, 3) float32 and
+    inertia factors (n, 3) float32 (ogjk_hull_moments)."""
+    import torch
+    n = int(nfaces.numel())
+    dev = points.device
+    mo = torch.empty((max(n, 1), 7), dtype=torch.float64, device=dev)
+    com = torch.empty((max(n, 1), 3), dtype=torch.float32, device=dev)
+    k = torch.empty((max(n, 1), 3), dtype=torch.float32, device=dev)
+    _check(lib().ogjk_hull_moments(points.data_ptr(), off.data_ptr(), faces.data_ptr(),
+                                   face_off.data_ptr(), nfaces.data_ptr(), n, mo.data_ptr(),
+                                   com.data_ptr(), k.data_ptr(), _torch_stream(stream)))
+    return mo[:n], com[:n], k[:n]
+
+
+def center_points(points, cloud, com, stream=None) -> None:
+    """points[i] -= com[cloud[i]] in place (ogjk_center_points)."""
+    _check(lib().ogjk_center_points(points.data_ptr(), cloud.data_ptr(), int(points.shape[0]),
+                                    com.data_ptr(), _torch_stream(stream)))
+
+
+def ingest_hulls(points, off, cnt, stream=None):
+    """The reference's ingestion of hull shapes for a batch of clouds
+    (visualization/main_opengl.cpp:114-121, 313): centre each cloud on its
+    hull's centre of mass, then the hull of the centred points (the faces it
+    renders) and that hull's inertia factors.  Returns a dict of device
+    tensors: points (centred copy), faces, face_off, nfaces, com, inertia_k,
+    volume."""
+    import torch
+    faces, face_off, nfaces = build_hulls_fit(points, off, cnt, stream)
+    _, com, _ = hull_moments(points, off, faces, face_off, nfaces, stream)
+    centred = points.clone()
+    n = int(cnt.numel())
+    cloud = torch.repeat_interleave(torch.arange(n, dtype=torch.int32, device=points.device),
+                                    cnt.long())
+    center_points(centred, cloud, com, stream)
+    faces2, face_off2, nfaces2 = build_hulls_fit(centred, off, cnt, stream)
+    mo2, _, k2 = hull_moments(centred, off, faces2, face_off2, nfaces2, stream)
+    return {"points": centred, "faces": faces2, "face_off": face_off2, "nfaces": nfaces2,
+            "com": com, "inertia_k": k2, "volume": mo2[:, 0]}

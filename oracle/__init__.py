@@ -262,3 +262,137 @@ def expand_pairs(body_pairs, sub_off, sub_cnt, max_sub: int | None = None):
     out = np.zeros((max(max_sub, 1), 2), np.int32)
     found = lib.oracle_expand_pairs(_p(bp), len(bp), _p(so), _p(sc), len(sc), _p(out), max_sub)
     return out[:min(found, max_sub)].copy(), int(found)
+
+
+# ------------------------------------------------------------ convex hulls
+
+LIBHULL = os.path.join(_HERE, "lib", "liboracle_hull.so")
+REFMESH = os.path.join(_HERE, "_ref", "libref_mesh.so")
+
+
+def _hull_lib():
+    if "hull" not in _libs:
+        if not os.path.exists(LIBHULL):
+            raise RuntimeError(f"oracle library missing: {LIBHULL} (run __graft_entry__.build())")
+        lib = C.CDLL(LIBHULL)
+        lib.oracle_build_hull.restype = C.c_int32
+        lib.oracle_build_hull.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]
+        lib.oracle_hull_moments.restype = None
+        lib.oracle_hull_moments.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
+        for f in ("oracle_hull_com", "oracle_hull_inertia_k"):
+            getattr(lib, f).restype = None
+            getattr(lib, f).argtypes = [C.c_void_p, C.c_void_p]
+        _libs["hull"] = lib
+    return _libs["hull"]
+
+
+def build_hull(points, cap: int | None = None):
+    """Incremental hull of (n, 3) float32 points: (nf, 3) int32 faces in the
+    reference's order, or None when cap is exceeded."""
+    p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+    cap = cap or max(4 * len(p), 8)
+    faces = np.zeros((cap, 3), np.int32)
+    nf = _hull_lib().oracle_build_hull(_p(p), len(p), _p(faces), cap)
+    return None if nf < 0 else faces[:nf].copy()
+
+
+def hull_moments(points, faces):
+    """(volume, 4V*centroid x, y, z, second moments x^2, y^2, z^2) in FP64."""
+    p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+    f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 3)
+    m = np.zeros(7, np.float64)
+    _hull_lib().oracle_hull_moments(_p(p), _p(f), len(f), _p(m))
+    return m
+
+
+def hull_com(m):
+    m = np.ascontiguousarray(m, np.float64)
+    c = np.zeros(3, np.float32)
+    _hull_lib().oracle_hull_com(_p(m), _p(c))
+    return c
+
+
+def hull_inertia_k(m):
+    m = np.ascontiguousarray(m, np.float64)
+    k = np.zeros(3, np.float32)
+    _hull_lib().oracle_hull_inertia_k(_p(m), _p(k))
+    return k
+
+
+def ref_mesh_available() -> bool:
+    return os.path.exists(REFMESH)
+
+
+class _AtlasVertex(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("normal", C.c_float * 3), ("uv", C.c_float * 2)]
+
+
+class _MeshAtlas(C.Structure):
+    _fields_ = [("vertices", C.POINTER(_AtlasVertex)), ("num_vertices", C.c_int),
+                ("vertex_cap", C.c_int), ("indices", C.c_void_p), ("num_indices", C.c_int),
+                ("index_cap", C.c_int), ("meshes", C.c_void_p), ("num_meshes", C.c_int),
+                ("mesh_cap", C.c_int)]
+
+
+def _ref_mesh():
+    if "refmesh" not in _libs:
+        lib = C.CDLL(REFMESH)
+        lib.atlas_add_convex_hull.restype = C.c_int
+        lib.atlas_add_convex_hull.argtypes = [C.POINTER(_MeshAtlas), C.c_void_p, C.c_int]
+        lib.atlas_init.argtypes = [C.POINTER(_MeshAtlas)]
+        lib.atlas_free.argtypes = [C.POINTER(_MeshAtlas)]
+        lib.center_hull_verts.argtypes = [C.c_void_p, C.c_int]
+        lib.compute_hull_inertia_k.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                               C.c_void_p]
+        lib.gen_gjk_random_hull.restype = C.POINTER(C.c_float)
+        lib.gen_gjk_random_hull.argtypes = [C.c_int, C.c_uint, C.POINTER(C.c_int)]
+        for g in ("gen_gjk_icosahedron", "gen_gjk_box", "gen_gjk_tetrahedron",
+                  "gen_gjk_octahedron"):
+            getattr(lib, g).restype = C.POINTER(C.c_float)
+            getattr(lib, g).argtypes = [C.POINTER(C.c_int)]
+        _libs["refmesh"] = lib
+    return _libs["refmesh"]
+
+
+def ref_hull_corners(points):
+    """The reference's hull as face-corner positions, (nf, 3, 3) float32, in its
+    face order (through atlas_add_convex_hull, which copies each face's
+    corners)."""
+    lib = _ref_mesh()
+    p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+    at = _MeshAtlas()
+    lib.atlas_init(C.byref(at))
+    try:
+        mid = lib.atlas_add_convex_hull(C.byref(at), _p(p), len(p))
+        if mid < 0:
+            return np.zeros((0, 3, 3), np.float32)
+        out = np.array([list(at.vertices[i].pos) for i in range(at.num_vertices)], np.float32)
+        return out.reshape(-1, 3, 3)
+    finally:
+        lib.atlas_free(C.byref(at))
+
+
+def ref_center(points):
+    lib = _ref_mesh()
+    p = np.array(points, dtype=np.float32).reshape(-1, 3)
+    lib.center_hull_verts(_p(p), len(p))
+    return p
+
+
+def ref_inertia_k(points):
+    lib = _ref_mesh()
+    p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+    k = np.zeros(3, np.float32)
+    lib.compute_hull_inertia_k(_p(p), len(p), _p(k[0:]), _p(k[1:]), _p(k[2:]))
+    return k
+
+
+def ref_gen(name: str, n: int = 0, seed: int = 0):
+    """The reference's own GJK vertex clouds (gen_gjk_*), (m, 3) float32."""
+    lib = _ref_mesh()
+    cnt = C.c_int()
+    ptr = (lib.gen_gjk_random_hull(n, seed, C.byref(cnt)) if name == "random_hull"
+           else getattr(lib, f"gen_gjk_{name}")(C.byref(cnt)))
+    out = np.ctypeslib.as_array(ptr, shape=(cnt.value * 3,)).reshape(-1, 3).copy()
+    C.CDLL(None).free(C.cast(ptr, C.c_void_p))
+    return out
