@@ -10,7 +10,7 @@ Workloads (--workload):
   noisy_sphere_large: 10,000 pairs of 100,000-vertex polytopes, unit-sphere
       points perturbed by N(0, 0.01^2) per coordinate, centres U[-1.5,1.5]^3
       (about 50% penetrate).
-  serving, broadphase, scene_step: see the functions of those names.
+  serving, broadphase, scene_step, hulls: see the functions of those names.
 Every pair runs GJK then EPA: FP32 vertex storage, FP32 support sweep with an
 exact FP64 re-rank, FP64 sub-distance and EPA arithmetic; the larger
 workloads sweep large bodies through their vertex clusters.
@@ -65,7 +65,8 @@ WORKLOADS = {
 
 
 # rocprofv3 summaries of the legs that are not in WORKLOADS
-LEG_PROFILES = {"broadphase": "broadphase_1m.md", "scene_step": "scene_step_20k.md"}
+LEG_PROFILES = {"broadphase": "broadphase_1m.md", "scene_step": "scene_step_20k.md",
+                "hulls": "hulls_100k.md"}
 
 
 def measured_traffic(workload: str):
@@ -451,12 +452,83 @@ def scene_step(args) -> int:
     return 0
 
 
+def hulls(args) -> int:
+    """Mesh-ingestion leg: 100,000 point clouds of 8-32 points uniform on the
+    unit sphere (the reference's random hull variants, MIN_HULL_VERTS ..
+    MAX_HULL_VERTS); a step ingests them all as the reference does each shape
+    (convex hull, centre of mass, centring, hull of the centred points and its
+    inertia factors).  Throughput in clouds/s."""
+    import torch
+    import oracle
+    import ogjk
+    from concurrent.futures import ThreadPoolExecutor
+    n = args.pairs or 100_000
+    rng = np.random.default_rng(5)
+    cnt = rng.integers(8, 33, n)
+    u = rng.normal(size=(int(cnt.sum()), 3))
+    u = (u / np.linalg.norm(u, axis=1, keepdims=True)).astype(np.float32)
+    off = np.zeros(n, np.int64)
+    off[1:] = np.cumsum(cnt)[:-1]
+    pts = torch.from_numpy(u).cuda()
+    doff = torch.from_numpy(off).cuda()
+    dcnt = torch.from_numpy(cnt.astype(np.int32)).cuda()
+    r = ogjk.ingest_hulls(pts, doff, dcnt)
+    torch.cuda.synchronize()
+    wall, ms = _event_steps(torch, lambda: ogjk.ingest_hulls(pts, doff, dcnt), args.steps,
+                            args.warmup)
+    # the hull kernel alone, for the roofline: points read once, faces written once
+    hw, hms = _event_steps(torch, lambda: ogjk.build_hulls(pts, doff, dcnt), args.steps,
+                           args.warmup)
+    nf = r["nfaces"].cpu().numpy()
+    alg = 12.0 * len(u) + 12.0 * float(nf.clip(min=0).sum()) + 16.0 * n
+    achieved = alg / (np.mean(hms) * 1e-3) / 1e9
+    line = {
+        "metric": "mesh ingestion clouds/s (convex hull, centring, inertia factors)",
+        "value": n / (wall / args.steps), "unit": "clouds/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"hull ingestion: {n} clouds of 8-32 points uniform on the unit "
+                               "sphere", "clouds": n, "points": int(len(u)),
+                   "faces": int(nf.clip(min=0).sum())},
+        "hull_kernel_ms": float(np.mean(hms)),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("hulls"),
+                     "kernel": "k_hull (one hull build of every cloud)",
+                     "kernel_ms_mean": float(np.mean(hms)), "algorithmic_bytes_per_launch": alg,
+                     "algorithmic_bytes": "12 B per point read, 12 B per face written, 16 B "
+                                          "per cloud of offsets and counts"},
+    }
+    if not args.no_cpu:
+        cores = min(len(os.sched_getaffinity(0)), 16)
+        t = time.perf_counter()
+        cen_o, faces_o, fo_o, nf_o, k_o = oracle.ingest_batch(u, off, cnt, nthreads=cores)
+        el = time.perf_counter() - t
+        m = n
+        faces = r["faces"].cpu().numpy()
+        fo = r["face_off"].cpu().numpy()
+        k = r["inertia_k"].cpu().numpy()
+        cen = r["points"].cpu().numpy()
+        same = [nf[i] == nf_o[i] and
+                np.array_equal(faces[fo[i]:fo[i] + nf[i]], faces_o[fo_o[i]:fo_o[i] + nf_o[i]])
+                for i in range(n)]
+        same = np.array(same) & (k.view(np.uint32) == k_o.view(np.uint32)).all(1)
+        cen_same = bool(np.array_equal(cen.view(np.uint32), cen_o.view(np.uint32)))
+        line["cpu_baseline"] = {
+            "value": m / el, "unit": "clouds/s", "cores": cores, "kind": "port",
+            "sample": f"all {m} clouds, C oracle of the reference mesh builder (the same "
+                      f"sequence), {cores} threads, {el:.2f}s"}
+        line["parity"] = {"clouds_checked": m, "bit_exact_fraction": float(np.mean(same)),
+                          "centred_points_identical": cen_same}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="sphere64", choices=sorted(WORKLOADS) + ["serving", "broadphase", "scene_step"])
+    ap.add_argument("--workload", default="sphere64", choices=sorted(WORKLOADS) + ["serving", "broadphase", "scene_step", "hulls"])
     ap.add_argument("--qps", type=int, default=1000, help="serving: request rate")
     ap.add_argument("--requests", type=int, default=2000, help="serving: requests per rate")
     ap.add_argument("--slots", type=int, default=8, help="serving: launches in flight")
@@ -496,6 +568,8 @@ def main() -> int:
         return broadphase(args)
     if args.workload == "scene_step":
         return scene_step(args)
+    if args.workload == "hulls":
+        return hulls(args)
     from ogjk import autotune
     spec = WORKLOADS[args.workload]
     lg, le = (int(x) for x in args.lanes.split(",")) if args.lanes else spec["lanes"]

@@ -279,6 +279,9 @@ def _hull_lib():
         lib.oracle_build_hull.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]
         lib.oracle_hull_moments.restype = None
         lib.oracle_hull_moments.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
+        lib.oracle_ingest_batch.restype = None
+        lib.oracle_ingest_batch.argtypes = [C.c_void_p] * 3 + [C.c_int32] + [C.c_void_p] * 5 + \
+            [C.c_int32]
         for f in ("oracle_hull_com", "oracle_hull_inertia_k"):
             getattr(lib, f).restype = None
             getattr(lib, f).argtypes = [C.c_void_p, C.c_void_p]
@@ -317,6 +320,26 @@ def hull_inertia_k(m):
     k = np.zeros(3, np.float32)
     _hull_lib().oracle_hull_inertia_k(_p(m), _p(k))
     return k
+
+
+def ingest_batch(points, off, cnt, nthreads: int = 1):
+    """The reference's ingestion of every cloud (hull, centring, hull of the
+    centred points, inertia factors) in C on nthreads threads.  Returns
+    (centred points, faces, face_off, nfaces, inertia_k); capacity 16 n + 64
+    faces per cloud."""
+    p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+    off = np.ascontiguousarray(off, dtype=np.int64)
+    cnt = np.ascontiguousarray(cnt, dtype=np.int32)
+    n = len(cnt)
+    fo = np.zeros(n + 1, np.int64)
+    fo[1:] = np.cumsum(16 * cnt.astype(np.int64) + 64)
+    faces = np.zeros((max(int(fo[-1]), 1), 3), np.int32)
+    nf = np.zeros(n, np.int32)
+    centred = np.zeros_like(p)
+    k = np.zeros((n, 3), np.float32)
+    _hull_lib().oracle_ingest_batch(_p(p), _p(off), _p(cnt), n, _p(faces), _p(fo), _p(nf),
+                                    _p(centred), _p(k), nthreads)
+    return centred, faces, fo, nf, k
 
 
 def ref_mesh_available() -> bool:

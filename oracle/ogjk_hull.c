@@ -196,3 +196,67 @@ void oracle_hull_inertia_k(const double* m, float* k) {
   k[1] = (float)((m[5] - m[0] * c1 * c1) / m[0]);
   k[2] = (float)((m[6] - m[0] * c2 * c2) / m[0]);
 }
+
+/* ---------------------------------------------------------------- batches */
+#include <pthread.h>
+
+typedef struct {
+  const float* pts;
+  const int64_t* off;
+  const int32_t* cnt;
+  int32_t n;
+  int32_t* faces;
+  const int64_t* face_off;
+  int32_t* nfaces;
+  float* centred;
+  float* k;
+  int32_t next;
+  pthread_mutex_t mu;
+} IngestJob;
+
+static void ingest_one(IngestJob* j, int32_t m) {
+  const float* p = j->pts + 3 * j->off[m];
+  float* pc = j->centred + 3 * j->off[m];
+  int32_t* f = j->faces + 3 * j->face_off[m];
+  const int32_t cap = (int32_t)(j->face_off[m + 1] - j->face_off[m]);
+  const int32_t n = j->cnt[m];
+  double mo[7];
+  float com[3];
+  int32_t nf = oracle_build_hull(p, n, f, cap);
+  oracle_hull_moments(p, f, nf > 0 ? nf : 0, mo);
+  oracle_hull_com(mo, com);
+  for (int32_t i = 0; i < n; i++)
+    for (int c = 0; c < 3; c++) pc[3 * i + c] = p[3 * i + c] - com[c];
+  nf = oracle_build_hull(pc, n, f, cap);
+  j->nfaces[m] = nf;
+  oracle_hull_moments(pc, f, nf > 0 ? nf : 0, mo);
+  oracle_hull_inertia_k(mo, j->k + 3 * m);
+}
+
+static void* ingest_worker(void* arg) {
+  IngestJob* j = (IngestJob*)arg;
+  for (;;) {
+    pthread_mutex_lock(&j->mu);
+    const int32_t m0 = j->next;
+    j->next += 64;
+    pthread_mutex_unlock(&j->mu);
+    if (m0 >= j->n) return NULL;
+    const int32_t m1 = m0 + 64 < j->n ? m0 + 64 : j->n;
+    for (int32_t m = m0; m < m1; m++) ingest_one(j, m);
+  }
+}
+
+/* The reference's ingestion of each cloud (hull, centre of mass, centring,
+ * hull of the centred points, inertia factors; main_opengl.cpp:114-121, 313)
+ * over n clouds on nthreads threads.  faces/nfaces hold the second hull. */
+void oracle_ingest_batch(const float* pts, const int64_t* off, const int32_t* cnt, int32_t n,
+                         int32_t* faces, const int64_t* face_off, int32_t* nfaces,
+                         float* centred, float* k, int32_t nthreads) {
+  IngestJob j = {pts, off, cnt, n, faces, face_off, nfaces, centred, k, 0,
+                 PTHREAD_MUTEX_INITIALIZER};
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  pthread_t th[256];
+  for (int t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, ingest_worker, &j);
+  for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+}
