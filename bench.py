@@ -638,6 +638,32 @@ def main() -> int:
     wall, ms, _ = timed(False, args.steps)
     print(f"# mixed: {wall / args.steps * 1e3:.3f} ms/step wall, kernel median {np.median(ms):.3f} ms",
           file=sys.stderr, flush=True)
+    # the same step captured once as a HIP graph (counter reset, partition,
+    # GJK per bucket, persistent EPA per class) and replayed: no host work per
+    # step; its outputs must equal the eager step's bit for bit
+    gout = scene.outputs(n, f64=False)
+    graph, _ = scene.capture(3, n, out=gout)
+    for _ in range(args.warmup):
+        graph.launch()
+    torch.cuda.synchronize()
+    gev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    t = time.perf_counter()
+    for a, b in gev:
+        a.record(stream)
+        graph.launch()
+        b.record(stream)
+    torch.cuda.synchronize()
+    g_wall = time.perf_counter() - t
+    g_ms = np.array([a.elapsed_time(b) for a, b in gev])
+    graph_info = {"ms_per_step": g_wall / args.steps * 1e3,
+                  "pairs_per_s": n * args.steps / g_wall,
+                  "step_ms_p50": float(np.percentile(g_ms, 50)),
+                  "identical_to_eager": bool(all(torch.equal(gout[k], timed_out[k])
+                                                 for k in ("simplices", "distances", "normals")))}
+    graph.close()
+    print(f"# graph replay: {graph_info['ms_per_step']:.3f} ms/step, identical to eager: "
+          f"{graph_info['identical_to_eager']}", file=sys.stderr, flush=True)
     plain = pipe64 = None
     if not args.no_plain:
         # comparison legs run fewer steps when a step is long (about 4 s each)
@@ -697,6 +723,7 @@ def main() -> int:
                    "clusters": {"min_verts": cmin, "build_s": build_s} if cmin else None,
                    "layout": "SoA fp32 scene; GJK kernel -> device worklist -> persistent EPA kernel"},
         "step_ms_p50": float(np.percentile(ms, 50)), "step_ms_p99": float(np.percentile(ms, 99)),
+        "graph": graph_info,
         "plain_f64": plain,
         "speedup_vs_plain_f64": (value / plain["pairs_per_s"]) if plain else None,
         "pipeline_f64_storage": pipe64,

@@ -52,12 +52,14 @@ def build_lib() -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     objdir = os.path.join(LIBDIR, "obj")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
-    for src in LIB_SOURCES:
-        obj = os.path.join(objdir, src.replace(".hip", ".o"))
-        _run([HIPCC, "-c", *DEVICE_FLAGS, f"-I{INC}", f"-I{CSRC}", "-o", obj,
-              os.path.join(CSRC, src)])
-        objs.append(obj)
+    objs = [os.path.join(objdir, src.replace(".hip", ".o")) for src in LIB_SOURCES]
+    # one hipcc per source, in parallel (the kernels file dominates)
+    from concurrent.futures import ThreadPoolExecutor
+    jobs = min(len(LIB_SOURCES), max(1, len(os.sched_getaffinity(0))))
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(lambda so: _run([HIPCC, "-c", *DEVICE_FLAGS, f"-I{INC}", f"-I{CSRC}", "-o",
+                                     so[1], os.path.join(CSRC, so[0])]),
+                    zip(LIB_SOURCES, objs)))
     out = os.path.join(LIBDIR, "libogjk.so")
     tmp = out + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
