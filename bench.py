@@ -531,7 +531,9 @@ def main() -> int:
     ap.add_argument("--workload", default="sphere64", choices=sorted(WORKLOADS) + ["serving", "broadphase", "scene_step", "hulls"])
     ap.add_argument("--qps", type=int, default=1000, help="serving: request rate")
     ap.add_argument("--requests", type=int, default=2000, help="serving: requests per rate")
-    ap.add_argument("--slots", type=int, default=8, help="serving: launches in flight")
+    ap.add_argument("--slots", type=int, default=4,
+                    help="serving: launches in flight (at most the process's hardware queues, "
+                         "4 by default: more slots share queues and block behind long launches)")
     ap.add_argument("--pairs", type=int, default=0, help="override the pair count")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-FP64 comparison")
