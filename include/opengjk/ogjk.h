@@ -293,7 +293,8 @@ ogjk_status ogjk_expand_pairs(const gkCollisionPair* d_body_pairs, int32_t n_bod
  * d_face_off[m + 1] - d_face_off[m] of them; d_face_off has n_clouds + 1
  * entries and d_face_off[n_clouds] == total_faces.  d_nfaces[m] = face count,
  * 0 when there is no hull (fewer than 4 points, collinear or coplanar), -1
- * when the capacity was too small, -2 for a non-finite coordinate.  The face
+ * when the capacity was too small, -2 for a non-finite coordinate, -3 when
+ * the cloud's face range is not inside [0, total_faces].  The face
  * list is the reference's, in its order, including its behaviour on dense
  * near-coplanar input (more than 2n - 4 faces can result: give about 4n).
  * Workspace: 12 * total_faces bytes.  Asynchronous.  Replaces build_hull

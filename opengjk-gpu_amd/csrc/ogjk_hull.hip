@@ -68,11 +68,13 @@ __device__ __forceinline__ void wave_argmin(float& v, int& i) {
 // One wave per cloud.  faces/scratch of cloud m start at 3 * face_off[m] and
 // hold face_off[m + 1] - face_off[m] faces; nfaces[m] = face count, 0 for no
 // hull (fewer than 4 points, collinear or coplanar), -1 when the capacity is
-// exceeded, -2 for a non-finite coordinate.
+// exceeded, -2 for a non-finite coordinate, -3 when the cloud's face range is
+// not inside [0, total_faces] (nothing is written for it).
 __global__ __launch_bounds__(kWave) void k_hull(const float* __restrict__ pts,
                                                 const int64_t* __restrict__ off,
                                                 const int32_t* __restrict__ cnt,
                                                 const int64_t* __restrict__ face_off,
+                                                int64_t total_faces,
                                                 int32_t* __restrict__ faces,
                                                 int32_t* __restrict__ nfaces,
                                                 int32_t* __restrict__ scratch) {
@@ -82,6 +84,10 @@ __global__ __launch_bounds__(kWave) void k_hull(const float* __restrict__ pts,
   const float* P = pts + 3 * off[m];
   const int64_t f0 = face_off[m];
   const int64_t cap64 = face_off[m + 1] - f0;
+  if (f0 < 0 || cap64 < 0 || face_off[m + 1] > total_faces) {
+    if (lane == 0) nfaces[m] = -3;
+    return;
+  }
   const int cap = cap64 < 0x7fffffff ? (int)cap64 : 0x7fffffff;
   int32_t* F = faces + 3 * f0;
   int32_t* vis = scratch + 3 * f0;  // cap entries
@@ -366,8 +372,8 @@ ogjk_status ogjk_build_hulls(const float* d_points, const int64_t* d_point_off,
   if (!d_workspace || workspace_bytes < need)
     return ogjk_fail(OGJK_ERR_INVALID, "workspace of %zu bytes needed", need);
   hipLaunchKernelGGL(k_hull, dim3((unsigned)n_clouds), dim3(kWave), 0, (hipStream_t)stream,
-                     d_points, d_point_off, d_point_cnt, d_face_off, d_faces, d_nfaces,
-                     (int32_t*)d_workspace);
+                     d_points, d_point_off, d_point_cnt, d_face_off, total_faces, d_faces,
+                     d_nfaces, (int32_t*)d_workspace);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? OGJK_OK : hip_err(e, "hull launch");
 }

@@ -163,3 +163,15 @@ def test_hull_edge_cases(torch_cuda):
                                                                    device="cuda"))
     torch.cuda.synchronize()
     assert nfaces.cpu().numpy().tolist() == [-1]
+    # a face range past total_faces is refused without writing
+    from ogjk import lib
+    face_off = torch.tensor([0, 100], dtype=torch.int64, device="cuda")
+    faces = torch.full((48, 3), -7, dtype=torch.int32, device="cuda")
+    nfaces = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ws = torch.empty(12 * 48, dtype=torch.uint8, device="cuda")
+    assert lib().ogjk_build_hulls(pts[-8:].contiguous().data_ptr(), (off[:1] * 0).data_ptr(),
+                                  cnt[-1:].contiguous().data_ptr(), 1, face_off.data_ptr(), 48,
+                                  faces.data_ptr(), nfaces.data_ptr(), ws.data_ptr(), ws.numel(),
+                                  None) == 0
+    torch.cuda.synchronize()
+    assert nfaces.item() == -3 and bool((faces == -7).all())
