@@ -547,7 +547,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(pairs are independent and would shard by contiguous ranges; see DESIGN.md)",
+              "(pairs are independent and would shard by contiguous, cost-balanced ranges of the "
+              "pair list, one process per GPU)",
               file=sys.stderr)
         return 2
 
