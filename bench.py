@@ -546,7 +546,7 @@ def main() -> int:
                          "default per workload)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(pairs are independent and would shard by contiguous, cost-balanced ranges of the "
               "pair list, one process per GPU)",
               file=sys.stderr)
