@@ -100,13 +100,26 @@ struct Faces {
   unsigned pend;   // bit q: degenerate, retire at the next iteration start
 };
 
-// Face normal and plane distance (compute_face_normal_distance,
-// GJK/cpu/EPA.c:92-129).  Returns false for a degenerate face.
-OGJK_DEV bool face_plane(const EpaShared& sh, int v0, int v1, int v2, D3& n, double& d) {
+// A new face (v0, v1, v2) in slot q: winding fixed against the centroid
+// (GJK/cpu/EPA.c:204-231, 791-819), then its normal and plane distance
+// (compute_face_normal_distance, GJK/cpu/EPA.c:92-129); degenerate faces are
+// marked for retirement at the next iteration.  The winding test's cross
+// product is reused for the plane: with no contraction, cross(p2 - p0,
+// p1 - p0) is exactly -cross(p1 - p0, p2 - p0) (products commute exactly,
+// x - y = -(y - x)), so the swapped face's normal is the negated one, bit for
+// bit.
+template <int FPL>
+OGJK_DEV void make_face_fixed(Faces<FPL>& F, int q, const EpaShared& sh, int v0, int v1, int v2,
+                              D3 centroid) {
   const D3 p0 = lds_v(sh, v0);
-  n = cross(sub(lds_v(sh, v1), p0), sub(lds_v(sh, v2), p0));
+  D3 n = cross(sub(lds_v(sh, v1), p0), sub(lds_v(sh, v2), p0));
+  const bool swap = dot(n, sub(centroid, p0)) > 0;
+  const int packed = swap ? (v0 | (v2 << 8) | (v1 << 16)) : (v0 | (v1 << 8) | (v2 << 16));
+  if (swap) n = neg(n);
+  double d;
   const double nsq = norm2(n);
-  if (nsq > kEps64 * kEps64) {
+  const bool ok = nsq > kEps64 * kEps64;
+  if (ok) {
     const double nn = sqrt(nsq);
     n = D3{n.x / nn, n.y / nn, n.z / nn};
     d = dot(n, p0);
@@ -114,26 +127,9 @@ OGJK_DEV bool face_plane(const EpaShared& sh, int v0, int v1, int v2, D3& n, dou
       n = neg(n);
       d = -d;
     }
-    return true;
+  } else {
+    d = 1e10;
   }
-  d = 1e10;
-  return false;
-}
-
-// Winding fix against the centroid (GJK/cpu/EPA.c:204-231, 791-819).
-OGJK_DEV int fix_winding(const EpaShared& sh, int v0, int v1, int v2, D3 centroid) {
-  const D3 p0 = lds_v(sh, v0);
-  const D3 n = cross(sub(lds_v(sh, v1), p0), sub(lds_v(sh, v2), p0));
-  if (dot(n, sub(centroid, p0)) > 0) return v0 | (v2 << 8) | (v1 << 16);
-  return v0 | (v1 << 8) | (v2 << 16);
-}
-
-template <int FPL>
-OGJK_DEV void make_face(Faces<FPL>& F, int q, const EpaShared& sh, int packed) {
-  const int v0 = packed & 0xff, v1 = (packed >> 8) & 0xff, v2 = (packed >> 16) & 0xff;
-  D3 n;
-  double d;
-  const bool ok = face_plane(sh, v0, v1, v2, n, d);
   F.nx[q] = n.x;
   F.ny[q] = n.y;
   F.nz[q] = n.z;
@@ -354,7 +350,7 @@ OGJK_DEV void epa(const BT& A, const BT& B, Simplex& s, double& distance, Wit& w
     const int a = lane == 3 ? 1 : 0;
     const int b = lane == 0 ? 1 : (lane == 1 ? 3 : (lane == 2 ? 2 : 3));
     const int c = lane == 0 ? 2 : (lane == 1 ? 1 : (lane == 2 ? 3 : 2));
-    make_face(F, 0, sh, fix_winding(sh, a, b, c, centroid));
+    make_face_fixed(F, 0, sh, a, b, c, centroid);
   }
 
   const double tol = kEps64 * 1e2;
@@ -465,7 +461,7 @@ OGJK_DEV void epa(const BT& A, const BT& B, Simplex& s, double& distance, Wit& w
     for (int q = 0; q < FPL; q++) {
       if (fr[q] && frank[q] < nsurv) {
         const int ed = sh.surv[frank[q]];
-        make_face(F, q, sh, fix_winding(sh, ed & 0xff, (ed >> 8) & 0xff, nid, centroid));
+        make_face_fixed(F, q, sh, ed & 0xff, (ed >> 8) & 0xff, nid, centroid);
       }
     }
   }
