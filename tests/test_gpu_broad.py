@@ -120,3 +120,11 @@ def test_expand_pairs_matches_oracle(torch_cuda):
     assert found == rfound and np.array_equal(got.cpu().numpy(), ref)
     got, found = ogjk.expand_pairs(T(bp), T(sub_off), T(sub_cnt), max_sub_pairs=rfound // 2)
     assert found == rfound and np.array_equal(got.cpu().numpy(), ref[:rfound // 2])
+
+
+def test_broad_phase_large_sparse_grid(torch_cuda):
+    """64M cells, 200K objects: most cells empty, start table and histogram far
+    larger than the object count."""
+    s = gen.spheres(200_000, seed=15, box=200.0, rmin=0.1, rmax=0.5, outside=0.01)
+    g = ogjk.Grid(1.0, 200.0, 400)
+    assert check_equal(torch_cuda, "sparse_400", s, g) > 200
