@@ -86,3 +86,31 @@ def test_scene_step_contacts_equal_abi_outputs(torch_cuda):
     assert found == found2 and torch.equal(pairs, pairs2) and n > nb // 2
     ref = as_records({k: v[:n] for k, v in out.items() if k != "_workspace" and v is not None})
     assert same_bits(con["contacts"][:n].cpu().numpy(), ref)
+
+
+def test_posed_bodies_invalid_pairs(torch_cuda):
+    """Out-of-range body indices in a per-body-pose pair list give NaN results
+    (gkSimplex outputs and contact records alike), the other pairs are
+    untouched by them."""
+    torch = torch_cuda
+    b = gen.make(gen.ELLIPSOID, 100, seed=65, box=0.0, nmin=8, nmax=64)
+    nb = len(b.cnt)
+    sc = ogjk.DeviceScene(b)
+    poses = torch.from_numpy(gen.body_poses(nb, seed=66, tbox=1.0)).cuda()
+    pairs = np.stack([np.arange(0, nb, 2), np.arange(1, nb, 2)], 1).astype(np.int32)
+    pairs[3, 0] = nb
+    pairs[7, 1] = -5
+    dp = torch.from_numpy(pairs).cuda()
+    out = sc.collide_posed_bodies(oracle.GJK_EPA, dp, poses)
+    con = sc.collide_posed_bodies_contacts(oracle.GJK_EPA, dp, poses)
+    ref = sc.collide_posed_bodies(oracle.GJK_EPA, dp[[0, 1, 2, 4]].contiguous(), poses)
+    torch.cuda.synchronize()
+    d = out["distances"].cpu().numpy()
+    c = con["contacts"][:len(pairs)].cpu().numpy()
+    simp = ogjk.simplices_from_device(out["simplices"])
+    assert np.isnan(d[[3, 7]]).all() and (simp["nvrtx"][[3, 7]] == 0).all()
+    assert np.isnan(c[[3, 7]]).all()
+    ok = np.ones(len(pairs), bool)
+    ok[[3, 7]] = False
+    assert np.isfinite(d[ok]).all() and np.isfinite(c[ok]).all()
+    assert np.array_equal(d[[0, 1, 2, 4]].view(np.uint32), ref["distances"].cpu().numpy().view(np.uint32))
