@@ -24,6 +24,10 @@ ogjk_status ogjk_fail(ogjk_status code, const char* fmt, ...);
 namespace {
 
 constexpr int kWave = 64;
+// A cloud whose face capacity is at most this works in LDS (faces, visible
+// list and horizon: 24 B per face, 6 KiB per wave) and copies its faces out
+// once; larger clouds work in the output buffer and the workspace.
+constexpr int kLdsFaces = 256;
 
 __device__ __forceinline__ float vol6(const float* __restrict__ p, int a, int b, int c,
                                       float qx, float qy, float qz) {
@@ -89,9 +93,11 @@ __global__ __launch_bounds__(kWave) void k_hull(const float* __restrict__ pts,
     return;
   }
   const int cap = cap64 < 0x7fffffff ? (int)cap64 : 0x7fffffff;
-  int32_t* F = faces + 3 * f0;
-  int32_t* vis = scratch + 3 * f0;  // cap entries
-  int32_t* hor = vis + cap64;       // 2 * cap entries
+  __shared__ int32_t lds[6 * kLdsFaces];
+  const bool in_lds = cap64 <= kLdsFaces;
+  int32_t* F = in_lds ? lds : faces + 3 * f0;                          // 3 * cap entries
+  int32_t* vis = in_lds ? lds + 3 * kLdsFaces : scratch + 3 * f0;     // cap entries
+  int32_t* hor = in_lds ? lds + 4 * kLdsFaces : scratch + 3 * f0 + cap64;  // 2 * cap
   if (n < 4) {
     if (lane == 0) nfaces[m] = 0;
     return;
@@ -274,6 +280,10 @@ __global__ __launch_bounds__(kWave) void k_hull(const float* __restrict__ pts,
     }
     nf = w + nh;
     __syncthreads();
+  }
+  if (in_lds) {
+    int32_t* out = faces + 3 * f0;
+    for (int i = lane; i < 3 * nf; i += kWave) out[i] = F[i];
   }
   if (lane == 0) nfaces[m] = nf;
 }
