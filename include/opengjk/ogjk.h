@@ -143,6 +143,21 @@ ogjk_status ogjk_collide_scene_ws(int mode, const ogjk_scene* scene, const int32
                                   float* d_normals, double* d_f64, void* d_workspace,
                                   size_t workspace_bytes, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): as ogjk_collide_scene, but the GJK -> EPA workspace
+ * (counters, bucketed pair list, EPA jobs) is a static device array of the
+ * library sized for OGJK_NOALLOC_MAX_PAIRS pairs (12.3 MiB, one per device,
+ * held until the process ends).  Calls no hipMalloc/hipFree.  More than
+ * OGJK_NOALLOC_MAX_PAIRS pairs is OGJK_ERR_INVALID.  Every call on one device
+ * shares that array (its counters are cleared by each launch), so calls that
+ * overlap on one device -- other streams, other threads -- race. */
+#define OGJK_NOALLOC_MAX_PAIRS 65536
+ogjk_status ogjk_collide_scene_noalloc(int mode, const ogjk_scene* scene, const int32_t* d_pairs,
+                                       int32_t n_pairs, gkSimplex* d_simplices,
+                                       float* d_distances, float* d_normals, double* d_f64,
+                                       void* stream);
+
 /* A GJK / GJK+EPA launch over a scene captured as one HIP graph: the whole
  * sequence (counter reset, size buckets, GJK, worklist, persistent EPA) is
  * replayed by a single ogjk_graph_launch, with no host work per batch.

@@ -192,6 +192,40 @@ ogjk_status run_staged(int mode, const HostScene& hs, const gkCollisionPair* pai
 
 }  // namespace
 
+// ------------------------------------------------- allocation-free variant
+//
+// The GJK -> EPA workspace of ogjk_collide_scene_noalloc: a static device
+// array sized for OGJK_NOALLOC_MAX_PAIRS pairs, one copy per device, held
+// until the process ends.  Host code never names `scratch` in a kernel
+// argument or copy (on the host it is only a stand-in for the symbol): its
+// device address comes from hipGetSymbolAddress, once per device, on that
+// device's first call.
+constexpr size_t kNoallocBytes = ogjk::workspace_bytes_for(OGJK_NOALLOC_MAX_PAIRS);
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+namespace {
+struct NoallocScratch {
+  std::mutex mu;
+  void* ptr[64] = {};
+};
+NoallocScratch g_noalloc;
+
+ogjk_status noalloc_scratch(void** out) {
+  int dev = 0;
+  OGJK_HIP(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= 64) return fail(OGJK_ERR_UNSUPPORTED, "device id %d", dev);
+  std::lock_guard<std::mutex> lock(g_noalloc.mu);
+  if (!g_noalloc.ptr[dev]) {
+    void* p = nullptr;
+    OGJK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(OGJK_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+    g_noalloc.ptr[dev] = p;
+  }
+  *out = g_noalloc.ptr[dev];
+  return OGJK_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int ogjk_version(void) { return 100; }
@@ -494,6 +528,22 @@ ogjk_status ogjk_collide_scene(int mode, const ogjk_scene* scene, const int32_t*
   }
   return ogjk_collide_scene_ws(mode, scene, d_pairs, n_pairs, d_simplices, d_distances, d_normals,
                                d_f64, ws, need, stream);
+}
+
+ogjk_status ogjk_collide_scene_noalloc(int mode, const ogjk_scene* scene, const int32_t* d_pairs,
+                                       int32_t n_pairs, gkSimplex* d_simplices,
+                                       float* d_distances, float* d_normals, double* d_f64,
+                                       void* stream) {
+  if (n_pairs > OGJK_NOALLOC_MAX_PAIRS)
+    return fail(OGJK_ERR_INVALID, "ogjk_collide_scene_noalloc: %d pairs exceed the maximum of %d",
+                n_pairs, (int)OGJK_NOALLOC_MAX_PAIRS);
+  const ogjk_status st = check_out(mode, n_pairs, d_simplices, d_distances, d_normals);
+  if (st != OGJK_OK || n_pairs == 0) return st;
+  void* ws = nullptr;
+  const ogjk_status s2 = noalloc_scratch(&ws);
+  if (s2 != OGJK_OK) return s2;
+  return ogjk_collide_scene_ws(mode, scene, d_pairs, n_pairs, d_simplices, d_distances, d_normals,
+                               d_f64, ws, kNoallocBytes, stream);
 }
 
 ogjk_status ogjk_collide_scene_f64(int mode, const ogjk_scene64* scene, const int32_t* d_pairs,

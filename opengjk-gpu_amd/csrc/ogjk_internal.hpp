@@ -75,7 +75,18 @@ struct BucketPlan {
 int set_config(const LaunchConfig& c);  // -1 if invalid
 LaunchConfig get_config();
 
-// Device workspace a GJK+EPA launch over n pairs needs (counters + worklist).
+// Device workspace a GJK+EPA launch over n pairs needs: 8 KiB of counters,
+// the bucketed pair list (4 B per pair), the per-block bucket counts of the
+// partition (4 B per bucket per 256 pairs) and one 192-B EPA job per pair,
+// each piece 256-B aligned.
+constexpr size_t kWsCounterBytes = 8192;
+constexpr size_t kEpaJobBytes = 192;
+constexpr size_t ws_up256(size_t b) { return (b + 255) & ~size_t(255); }
+constexpr size_t workspace_bytes_for(int n) {
+  return kWsCounterBytes + ws_up256(4 * (size_t)(n > 0 ? n : 0)) +
+         ws_up256(4 * (size_t)kMaxBuckets * (size_t)(((n > 0 ? n : 0) + 255) / 256)) +
+         (size_t)(n > 0 ? n : 0) * kEpaJobBytes;
+}
 size_t workspace_bytes(int n);
 
 // ws: device workspace of workspace_bytes(n) (used by mode 3 only).

@@ -275,7 +275,7 @@ struct EpaJob {
   int ia[4], ib[4];
   int n, pair;
 };
-static_assert(sizeof(EpaJob) == 192, "EpaJob layout");
+static_assert(sizeof(EpaJob) == kEpaJobBytes, "EpaJob layout");
 
 OGJK_DEV void job_store(EpaJob* j, int pair, const Simplex& s, const Wit& w, double dist) {
 #pragma unroll
@@ -387,7 +387,8 @@ constexpr int kSlices = 8;
 constexpr int kLineU = 32;  // uints per 128-B line
 constexpr int kClaimBase = 256;
 constexpr int kQueueEpa = 4;
-constexpr size_t kCounterBytes = 8192;  // = (kClaimBase + 7*kSlices*kLineU) * 4
+constexpr size_t kCounterBytes = kWsCounterBytes;  // = (kClaimBase + 7*kSlices*kLineU) * 4
+static_assert(kCounterBytes == (kClaimBase + 7 * kSlices * kLineU) * 4, "counter block");
 
 // Claim the next item of [0, count) for every lane with `want` (group
 // leaders).  Returns the item, or 0xffffffff when all slices are exhausted.
@@ -705,8 +706,11 @@ static size_t ws_jobs_off(int n) {
 
 size_t workspace_bytes(int n) {
   const int m = n > 0 ? n : 0;
-  return ws_jobs_off(m) + (size_t)m * sizeof(EpaJob);
+  return ws_jobs_off(m) + (size_t)m * sizeof(EpaJob);  // == workspace_bytes_for(m)
 }
+static_assert(workspace_bytes_for(65536) == kCounterBytes + 4 * 65536 + 4 * kMaxBuckets * 256 +
+                                               65536 * sizeof(EpaJob),
+              "workspace layout");
 
 template <int LPP, int MODE, class Src>
 static hipError_t launch_pairs(const Src& src, const OutArgs& o, int n, hipStream_t st) {

@@ -46,7 +46,7 @@ C_ABI_SYMBOLS = [
     "ogjk_server_info", "ogjk_server_destroy", "ogjk_broad_phase_workspace", "ogjk_broad_phase",
     "ogjk_expand_pairs_workspace", "ogjk_expand_pairs", "ogjk_collide_scene_contacts",
     "ogjk_collide_posed_bodies_contacts", "ogjk_build_hulls", "ogjk_hull_moments",
-    "ogjk_center_points",
+    "ogjk_center_points", "ogjk_collide_scene_noalloc",
 ]
 
 
@@ -125,6 +125,8 @@ def lib():
             "ogjk_graph_destroy": (C.c_int, [vp]),
             "ogjk_build_clusters": (C.c_int, [C.POINTER(_Scene), C.POINTER(_Clusters), vp]),
             "ogjk_collide_scene": (C.c_int, [C.c_int, C.POINTER(_Scene), vp, i32, vp, vp, vp, vp, vp]),
+            "ogjk_collide_scene_noalloc": (C.c_int, [C.c_int, C.POINTER(_Scene), vp, i32, vp, vp,
+                                                     vp, vp, vp]),
             "ogjk_collide_scene_f64": (C.c_int, [C.c_int, C.POINTER(_Scene64), vp, i32, vp, vp, vp, vp, vp]),
             "ogjk_collide_polytopes": (C.c_int, [C.c_int, vp, vp, i32, vp, vp, vp, vp]),
             "ogjk_collide_polytopes_indexed": (C.c_int, [C.c_int, vp, i32, vp, i32, vp, vp, vp, vp]),
