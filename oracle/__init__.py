@@ -49,6 +49,8 @@ def _lib(bits: int):
         lib = C.CDLL(path)
         lib.oracle_batch.restype = None
         lib.oracle_batch.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 9 + [C.c_int]
+        lib.oracle_batch_f32.restype = None
+        lib.oracle_batch_f32.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 9 + [C.c_int]
         lib.oracle_real_size.restype = C.c_int
         assert lib.oracle_real_size() == bits // 8
         _libs[bits] = lib
@@ -71,12 +73,16 @@ def run(mode: int, coords, off, cnt, pa, pb, bits: int = 64, nthreads: int = 1,
         simplices=None, distances=None, stats: bool = False) -> Result:
     """Run the oracle over pairs (pa[i], pb[i]) of bodies in an AoS pool.
 
-    ``coords`` is an (N, 3) float32 pool; it is promoted exactly to the oracle's
-    precision.  For mode EPA, ``simplices``/``distances`` are the inputs (copied).
+    ``coords`` is an (N, 3) pool.  A float32 pool is promoted exactly to the
+    oracle's precision pair by pair inside the workers (a large pool is never
+    converted as a whole); a pool already in the oracle's precision is read
+    in place.  For mode EPA, ``simplices``/``distances`` are the inputs
+    (copied).
     """
     real = np.float64 if bits == 64 else np.float32
     n = len(pa)
-    c = np.ascontiguousarray(coords, dtype=real).reshape(-1, 3)
+    per_pair = np.asarray(coords).dtype == np.float32 and real != np.float32
+    c = np.ascontiguousarray(coords, dtype=np.float32 if per_pair else real).reshape(-1, 3)
     off = np.ascontiguousarray(off, dtype=np.int64)
     cnt = np.ascontiguousarray(cnt, dtype=np.int32)
     pa = np.ascontiguousarray(pa, dtype=np.int32)
@@ -90,9 +96,30 @@ def run(mode: int, coords, off, cnt, pa, pb, bits: int = 64, nthreads: int = 1,
     normals = np.zeros((n, 3), dtype=real) if mode & EPA else None
     st = np.zeros(n, dtype=STATS_DTYPE) if stats else None
     if n:
-        _lib(bits).oracle_batch(mode, n, _p(c), _p(off), _p(cnt), _p(pa), _p(pb), _p(simp),
-                                _p(dist), _p(normals), _p(st), int(nthreads))
+        fn = _lib(bits).oracle_batch_f32 if per_pair else _lib(bits).oracle_batch
+        fn(mode, n, _p(c), _p(off), _p(cnt), _p(pa), _p(pb), _p(simp), _p(dist), _p(normals),
+           _p(st), int(nthreads))
     return Result(dist, simp, normals, st)
+
+
+def compact(coords, off, cnt, pa, pb, dtype=np.float64):
+    """The bodies the pairs (pa[i], pb[i]) name, gathered into a pool of their
+    own in `dtype` (exact promotion): (coords, off, cnt, pa, pb) with the
+    pairs renumbered.  Lets a timed oracle run read a pool of only the bodies
+    it uses, already in its precision."""
+    pa = np.asarray(pa, np.int64)
+    pb = np.asarray(pb, np.int64)
+    bodies, inv = np.unique(np.concatenate([pa, pb]), return_inverse=True)
+    cnts = np.asarray(cnt, np.int64)[bodies]
+    offs = np.zeros(len(bodies), np.int64)
+    if len(bodies):
+        offs[1:] = np.cumsum(cnts)[:-1]
+    total = int(cnts.sum())
+    src = np.repeat(np.asarray(off, np.int64)[bodies] - offs, cnts) + np.arange(total)
+    pool = np.asarray(coords).reshape(-1, 3)[src].astype(dtype)
+    n = len(pa)
+    return (pool, offs, cnts.astype(np.int32), inv[:n].astype(np.int32),
+            inv[n:].astype(np.int32))
 
 
 # ----------------------------------------------------------------- reference

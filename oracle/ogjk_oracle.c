@@ -1185,16 +1185,39 @@ typedef struct {
   oreal* normals;
   ostats* stats;
   int next; /* next unclaimed pair (atomic) */
+  const float* coords32; /* float pool promoted per pair (oracle_batch_f32), or NULL */
 } batch_job;
 
-static void run_pair(batch_job* J, int i) {
+/* Per-worker buffer of the two bodies of one pair, promoted from float. */
+typedef struct {
+  oreal* v;
+  int64_t cap;
+} pair_buf;
+
+static const oreal* promote(pair_buf* B, const float* src, int64_t n, int64_t at) {
+  for (int64_t k = 0; k < 3 * n; k++) B->v[3 * at + k] = (oreal)src[k];
+  return B->v + 3 * at;
+}
+
+static void run_pair_buf(batch_job* J, int i, pair_buf* B) {
   opoly b1, b2;
   memset(&b1, 0, sizeof(b1));
   memset(&b2, 0, sizeof(b2));
   b1.numpoints = J->cnt[J->pa[i]];
-  b1.coord = J->coords + 3 * J->off[J->pa[i]];
   b2.numpoints = J->cnt[J->pb[i]];
-  b2.coord = J->coords + 3 * J->off[J->pb[i]];
+  if (J->coords32) {
+    const int64_t n1 = b1.numpoints > 0 ? b1.numpoints : 0, n2 = b2.numpoints > 0 ? b2.numpoints : 0;
+    if (n1 + n2 > B->cap) {
+      free(B->v);
+      B->cap = 2 * (n1 + n2);
+      B->v = (oreal*)malloc((size_t)(3 * B->cap) * sizeof(oreal));
+    }
+    b1.coord = promote(B, J->coords32 + 3 * J->off[J->pa[i]], n1, 0);
+    b2.coord = promote(B, J->coords32 + 3 * J->off[J->pb[i]], n2, n1);
+  } else {
+    b1.coord = J->coords + 3 * J->off[J->pa[i]];
+    b2.coord = J->coords + 3 * J->off[J->pb[i]];
+  }
   ostats* st = J->stats ? &J->stats[i] : NULL;
   if (st) memset(st, 0, sizeof(*st));
   if (J->mode & 1) {
@@ -1208,37 +1231,54 @@ static void run_pair(batch_job* J, int i) {
 
 static void* batch_worker(void* arg) {
   batch_job* J = (batch_job*)arg;
+  pair_buf B = {NULL, 0};
   const int chunk = 32;
   for (;;) {
     const int start = __atomic_fetch_add(&J->next, chunk, __ATOMIC_RELAXED);
     if (start >= J->n) break;
     const int end = start + chunk < J->n ? start + chunk : J->n;
-    for (int i = start; i < end; i++) run_pair(J, i);
+    for (int i = start; i < end; i++) run_pair_buf(J, i, &B);
   }
+  free(B.v);
   return NULL;
 }
 
-void oracle_batch(int mode, int n, const oreal* coords, const int64_t* off, const int32_t* cnt,
-                  const int32_t* pair_a, const int32_t* pair_b, osimplex* simplices,
-                  oreal* distances, oreal* normals, ostats* stats, int nthreads) {
-  if (n <= 0) return;
-  batch_job J = {mode, n, coords, off, cnt, pair_a, pair_b, simplices, distances, normals, stats, 0};
+static void run_batch(batch_job* J, int nthreads) {
   if (nthreads <= 1) {
-    for (int i = 0; i < n; i++) run_pair(&J, i);
+    batch_worker(J);
     return;
   }
   if (nthreads > 256) nthreads = 256;
   pthread_t th[256];
   int started = 0;
   for (int t = 0; t < nthreads; t++) {
-    if (pthread_create(&th[t], NULL, batch_worker, &J) != 0) break;
+    if (pthread_create(&th[t], NULL, batch_worker, J) != 0) break;
     started++;
   }
   if (started == 0) {
-    batch_worker(&J);
+    batch_worker(J);
     return;
   }
   for (int t = 0; t < started; t++) pthread_join(th[t], NULL);
+}
+
+void oracle_batch(int mode, int n, const oreal* coords, const int64_t* off, const int32_t* cnt,
+                  const int32_t* pair_a, const int32_t* pair_b, osimplex* simplices,
+                  oreal* distances, oreal* normals, ostats* stats, int nthreads) {
+  if (n <= 0) return;
+  batch_job J = {mode, n, coords, off, cnt, pair_a, pair_b, simplices, distances, normals, stats,
+                 0, NULL};
+  run_batch(&J, nthreads);
+}
+
+void oracle_batch_f32(int mode, int n, const float* coords, const int64_t* off,
+                      const int32_t* cnt, const int32_t* pair_a, const int32_t* pair_b,
+                      osimplex* simplices, oreal* distances, oreal* normals, ostats* stats,
+                      int nthreads) {
+  if (n <= 0) return;
+  batch_job J = {mode, n, NULL, off, cnt, pair_a, pair_b, simplices, distances, normals, stats,
+                 0, coords};
+  run_batch(&J, nthreads);
 }
 
 int oracle_real_size(void) { return (int)sizeof(oreal); }

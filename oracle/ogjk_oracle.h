@@ -76,6 +76,14 @@ void oracle_batch(int mode, int n, const oreal* coords, const int64_t* off,
                   osimplex* simplices, oreal* distances, oreal* normals,
                   ostats* stats, int nthreads);
 
+/* As oracle_batch over a float32 AoS pool: each worker promotes the two
+ * bodies of a pair exactly to oreal before running it, so a large pool is
+ * never converted as a whole. */
+void oracle_batch_f32(int mode, int n, const float* coords, const int64_t* off,
+                      const int32_t* cnt, const int32_t* pair_a, const int32_t* pair_b,
+                      osimplex* simplices, oreal* distances, oreal* normals,
+                      ostats* stats, int nthreads);
+
 int oracle_real_size(void);
 
 #ifdef __cplusplus
