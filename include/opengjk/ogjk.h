@@ -79,7 +79,9 @@ typedef struct ogjk_scene {
   const ogjk_clusters* clusters; /* host pointer, or NULL: no culling */
 } ogjk_scene;
 
-/* Plain-FP64 scene (double coordinates) for the comparison path. */
+/* Plain-FP64 scene (double coordinates) for the comparison path: SoA x/y/z,
+ * or, when xyz is not NULL, one AoS pool [x0, y0, z0, x1, ...] (the
+ * reference's gkPolytope coordinate layout at FP64) and x/y/z are ignored. */
 typedef struct ogjk_scene64 {
   const double* x;
   const double* y;
@@ -87,6 +89,7 @@ typedef struct ogjk_scene64 {
   const int64_t* body_off;
   const int32_t* body_cnt;
   int32_t n_bodies;
+  const double* xyz;
 } ogjk_scene64;
 
 /* Per-pair FP64 record for verification (optional, may be NULL):
@@ -102,9 +105,13 @@ int ogjk_device_count(void);
 ogjk_status ogjk_sync(void* stream);
 /* Launch shape, process-wide, read at launch time: lanes of a wave64 per
  * pair in the GJK kernel (8, 16, 32, 64; default 8) and in the EPA kernel
- * (16, 32, 64; default 32); fused != 0 runs GJK+EPA as one kernel instead of
- * GJK -> device worklist -> persistent EPA.  Not synchronised with launches
- * issued concurrently from other threads. */
+ * (16, 32, 64; default 32).  fused 0: GJK -> device worklist -> persistent
+ * EPA; 1: GJK+EPA as one kernel, one group per pair; 2: the reference's
+ * launch shape for the plain-FP64 comparison path (ogjk_collide_scene_f64):
+ * a GJK kernel over every pair, then an EPA kernel over every pair, one group
+ * per pair each, non-persistent (GJK/gpu/openGJK.cu:2968-3004), where GJK
+ * lanes may also be 1 (one thread per pair); other calls run as fused 0.
+ * Not synchronised with launches issued concurrently from other threads. */
 ogjk_status ogjk_set_config(int gjk_lanes, int epa_lanes, int fused);
 
 /* Size-bucketed launch shape: pairs with max(nA, nB) <= max_verts[b] (and
@@ -355,6 +362,9 @@ ogjk_status ogjk_build_clusters(const ogjk_scene* scene, ogjk_clusters* clusters
  * mixed-precision speed-up). */
 ogjk_status ogjk_pack_scene_f64(const float* d_coords_aos, int64_t n_verts, double* d_x,
                                 double* d_y, double* d_z, void* stream);
+/* The same pool as an FP64 AoS array of 3 * n_verts doubles (ogjk_scene64.xyz). */
+ogjk_status ogjk_pack_scene_f64_aos(const float* d_coords_aos, int64_t n_verts, double* d_xyz,
+                                    void* stream);
 ogjk_status ogjk_collide_scene_f64(int mode, const ogjk_scene64* scene, const int32_t* d_pairs,
                                    int32_t n_pairs, gkSimplex* d_simplices, float* d_distances,
                                    float* d_normals, double* d_f64, void* stream);

@@ -243,10 +243,11 @@ ogjk_status ogjk_set_config(int gjk_lanes, int epa_lanes, int fused) {
   c.nb = 1;
   c.gjk_lanes[0] = gjk_lanes;
   c.epa_lanes[0] = epa_lanes;
-  c.fused = fused ? 1 : 0;
+  c.fused = fused;
   if (ogjk::set_config(c) != 0)
-    return fail(OGJK_ERR_INVALID, "lanes per pair: GJK 8/16/32/64, EPA 16/32/64 (got %d, %d)",
-                gjk_lanes, epa_lanes);
+    return fail(OGJK_ERR_INVALID,
+                "lanes per pair: GJK 8/16/32/64 (1 with fused 2), EPA 16/32/64; fused 0, 1 or 2 "
+                "(got %d, %d, %d)", gjk_lanes, epa_lanes, fused);
   return OGJK_OK;
 }
 
@@ -283,6 +284,15 @@ ogjk_status ogjk_pack_scene(const float* d_coords_aos, const int64_t* d_body_off
     return fail(OGJK_ERR_INVALID, "NULL body table pointer");
   OGJK_HIP(ogjk::launch_pack(d_coords_aos, d_body_off, d_body_cnt, n_bodies, n_verts, d_x, d_y,
                              d_z, d_r1, (hipStream_t)stream),
+           "pack launch");
+  return OGJK_OK;
+}
+
+ogjk_status ogjk_pack_scene_f64_aos(const float* d_coords_aos, int64_t n_verts, double* d_xyz,
+                                    void* stream) {
+  if (n_verts < 0) return fail(OGJK_ERR_INVALID, "negative size");
+  if (n_verts > 0 && (!d_coords_aos || !d_xyz)) return fail(OGJK_ERR_INVALID, "NULL pointer");
+  OGJK_HIP(ogjk::launch_pack_f64_aos(d_coords_aos, n_verts, d_xyz, (hipStream_t)stream),
            "pack launch");
   return OGJK_OK;
 }
@@ -552,13 +562,15 @@ ogjk_status ogjk_collide_scene_f64(int mode, const ogjk_scene64* scene, const in
   const ogjk_status st = check_out(mode, n_pairs, d_simplices, d_distances, d_normals);
   if (st != OGJK_OK) return st;
   if (n_pairs == 0) return OGJK_OK;
-  if (!scene || !scene->x || !scene->y || !scene->z || !scene->body_off || !scene->body_cnt)
+  if (!scene || (!scene->xyz && (!scene->x || !scene->y || !scene->z)) || !scene->body_off ||
+      !scene->body_cnt)
     return fail(OGJK_ERR_INVALID, "incomplete scene");
   if (!d_pairs && (int64_t)scene->n_bodies < 2 * (int64_t)n_pairs)
     return fail(OGJK_ERR_INVALID, "implicit pairs need %lld bodies, scene has %d",
                 2 * (long long)n_pairs, scene->n_bodies);
-  const ogjk::SceneArgs sa{scene->x, scene->y, scene->z, scene->body_off, scene->body_cnt,
-                           nullptr, scene->n_bodies, d_pairs};
+  ogjk::SceneArgs sa{scene->x, scene->y, scene->z, scene->body_off, scene->body_cnt,
+                     nullptr, scene->n_bodies, d_pairs};
+  sa.xyz = scene->xyz;
   const ogjk::OutArgs oa{d_simplices, d_distances, d_normals, d_f64};
   void* ws = nullptr;
   if (mode != OGJK_MODE_EPA) {

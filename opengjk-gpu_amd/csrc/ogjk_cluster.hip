@@ -143,9 +143,9 @@ __global__ void __launch_bounds__(256) k_cluster_bounds(const float* __restrict_
                                                         const float* __restrict__ cy,
                                                         const float* __restrict__ cz,
                                                         const int32_t* __restrict__ orig,
-                                                        int64_t n_clusters,
+                                                        int64_t c0, int64_t n_clusters,
                                                         ClusterBound* __restrict__ bounds) {
-  const int64_t c = ((int64_t)blockIdx.x * 256 + threadIdx.x) / kCluster;
+  const int64_t c = c0 + ((int64_t)blockIdx.x * 256 + threadIdx.x) / kCluster;
   const int t = threadIdx.x & (kCluster - 1);
   const bool in = c < n_clusters;
   const int64_t s = c * kCluster + t;
@@ -409,10 +409,11 @@ hipError_t build_clusters(const float* x, const float* y, const float* z, const 
     }
   }
   const int64_t n_clusters = total / kCluster;
-  if (n_clusters > 0) {
-    const int64_t threads = n_clusters * kCluster;
-    hipLaunchKernelGGL(k_cluster_bounds, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st,
-                       cx, cy, cz, orig, n_clusters, (ClusterBound*)bounds);
+  // 2^24 clusters (2^29 work-items) per dispatch: a dispatch holds < 2^32
+  for (int64_t c0 = 0; c0 < n_clusters; c0 += (int64_t)1 << 24) {
+    const int64_t nc = n_clusters - c0 < ((int64_t)1 << 24) ? n_clusters - c0 : ((int64_t)1 << 24);
+    hipLaunchKernelGGL(k_cluster_bounds, dim3((unsigned)((nc * kCluster + 255) / 256)), dim3(256),
+                       0, st, cx, cy, cz, orig, c0, n_clusters, (ClusterBound*)bounds);
   }
   hipLaunchKernelGGL(k_mark_super, dim3((unsigned)((n_bodies + 255) / 256)), dim3(256), 0, st, cnt,
                      body_slot, body_super, n_bodies, (ClusterBound*)bounds);

@@ -1,9 +1,12 @@
 // opengjk-amd device EPA for one pair on a group of LPP lanes (LPP <= 64).
 //
 // Layout: the polytope's vertex pool (<= 132 FP64 points and their body
-// indices) and the horizon edge lists live in LDS; the 128 face slots are
-// spread over the group's lanes, slot = q*LPP + lane, FPL = 128/LPP faces per
-// lane held in registers (normal, plane distance, packed vertex ids).
+// indices), the horizon edge lists and the input witnesses live in LDS; the
+// 128 face slots are spread over the group's lanes, slot = q*LPP + lane,
+// FPL = 128/LPP faces per lane held in registers (normal, plane distance,
+// packed vertex ids).  Nothing else of the pair stays in registers through
+// the expansion loop: the starting tetrahedron is slots 0-3 of the vertex
+// pool and is read back from there when the loop ends.
 //
 // The reference EPA (GJK/cpu/EPA.c:362-864) is sequential; its discrete
 // choices are reproduced exactly with ballots:
@@ -27,8 +30,11 @@ constexpr int kEpaMaxVerts = kEpaMaxFaces + 4;
 struct EpaShared {
   double vx[kEpaMaxVerts], vy[kEpaMaxVerts], vz[kEpaMaxVerts];
   int via[kEpaMaxVerts], vib[kEpaMaxVerts];
-  unsigned short edir[kEpaMaxFaces * 3];  // directed edge v1 | v2 << 8
+  // edges of the faces removed this iteration: directed edge v1 | v2 << 8 in
+  // the low half, its direction-free key in the high half
+  unsigned edir[kEpaMaxFaces * 3];
   unsigned short surv[kEpaMaxFaces * 3];  // surviving horizon edges, in order
+  double w_in[6];                         // the witnesses EPA started from
 };
 
 OGJK_DEV void wave_sync() {
@@ -36,10 +42,11 @@ OGJK_DEV void wave_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// Direction-free key of a packed edge (v1 | v2 << 8).
-OGJK_DEV int undirected(int e) {
-  const int a = e & 0xff, b = e >> 8;
-  return a < b ? (a | (b << 8)) : (b | (a << 8));
+// Directed edge (a, b) packed as a | b << 8, with its direction-free key
+// (min | max << 8) in the high 16 bits.
+OGJK_DEV unsigned edge_word(int a, int b) {
+  const unsigned lo = a < b ? (unsigned)(a | (b << 8)) : (unsigned)(b | (a << 8));
+  return (unsigned)(a | (b << 8)) | (lo << 16);
 }
 
 OGJK_DEV D3 lds_v(const EpaShared& sh, int i) { return D3{sh.vx[i], sh.vy[i], sh.vz[i]}; }
@@ -100,43 +107,55 @@ struct Faces {
   unsigned pend;   // bit q: degenerate, retire at the next iteration start
 };
 
-// A new face (v0, v1, v2) in slot q: winding fixed against the centroid
+// A built face: unit normal, plane distance, packed vertex ids, and whether
+// it is degenerate (retired at the next iteration start).
+struct NewFace {
+  D3 n;
+  double d;
+  int packed;
+  bool ok;
+};
+
+// A new face (v0, v1, v2): winding fixed against the centroid
 // (GJK/cpu/EPA.c:204-231, 791-819), then its normal and plane distance
-// (compute_face_normal_distance, GJK/cpu/EPA.c:92-129); degenerate faces are
-// marked for retirement at the next iteration.  The winding test's cross
-// product is reused for the plane: with no contraction, cross(p2 - p0,
+// (compute_face_normal_distance, GJK/cpu/EPA.c:92-129).  The winding test's
+// cross product is reused for the plane: with no contraction, cross(p2 - p0,
 // p1 - p0) is exactly -cross(p1 - p0, p2 - p0) (products commute exactly,
 // x - y = -(y - x)), so the swapped face's normal is the negated one, bit for
 // bit.
-template <int FPL>
-OGJK_DEV void make_face_fixed(Faces<FPL>& F, int q, const EpaShared& sh, int v0, int v1, int v2,
-                              D3 centroid) {
+OGJK_DEV NewFace build_face(const EpaShared& sh, int v0, int v1, int v2, D3 centroid) {
   const D3 p0 = lds_v(sh, v0);
   D3 n = cross(sub(lds_v(sh, v1), p0), sub(lds_v(sh, v2), p0));
   const bool swap = dot(n, sub(centroid, p0)) > 0;
-  const int packed = swap ? (v0 | (v2 << 8) | (v1 << 16)) : (v0 | (v1 << 8) | (v2 << 16));
+  NewFace f;
+  f.packed = swap ? (v0 | (v2 << 8) | (v1 << 16)) : (v0 | (v1 << 8) | (v2 << 16));
   if (swap) n = neg(n);
-  double d;
   const double nsq = norm2(n);
-  const bool ok = nsq > kEps64 * kEps64;
-  if (ok) {
+  f.ok = nsq > kEps64 * kEps64;
+  if (f.ok) {
     const double nn = sqrt(nsq);
     n = D3{n.x / nn, n.y / nn, n.z / nn};
-    d = dot(n, p0);
-    if (d < 0) {
+    f.d = dot(n, p0);
+    if (f.d < 0) {
       n = neg(n);
-      d = -d;
+      f.d = -f.d;
     }
   } else {
-    d = 1e10;
+    f.d = 1e10;
   }
-  F.nx[q] = n.x;
-  F.ny[q] = n.y;
-  F.nz[q] = n.z;
-  F.dist[q] = d;
-  F.verts[q] = packed;
+  f.n = n;
+  return f;
+}
+
+template <int FPL>
+OGJK_DEV void put_face(Faces<FPL>& F, int q, const NewFace& f) {
+  F.nx[q] = f.n.x;
+  F.ny[q] = f.n.y;
+  F.nz[q] = f.n.z;
+  F.dist[q] = f.d;
+  F.verts[q] = f.packed;
   F.valid |= 1u << q;
-  if (!ok) F.pend |= 1u << q;
+  if (!f.ok) F.pend |= 1u << q;
 }
 
 // Exclusive rank of set flags in slot order (slot = q*LPP + lane).
@@ -328,6 +347,14 @@ OGJK_DEV void epa(const BT& A, const BT& B, Simplex& s, double& distance, Wit& w
     sh.via[lane] = sa(s, lane);
     sh.vib[lane] = sb(s, lane);
   }
+  if (lane == 0) {
+    sh.w_in[0] = w.w0.x;
+    sh.w_in[1] = w.w0.y;
+    sh.w_in[2] = w.w0.z;
+    sh.w_in[3] = w.w1.x;
+    sh.w_in[4] = w.w1.y;
+    sh.w_in[5] = w.w1.z;
+  }
   wave_sync();
   D3 centroid{0.0, 0.0, 0.0};
 #pragma unroll
@@ -350,12 +377,13 @@ OGJK_DEV void epa(const BT& A, const BT& B, Simplex& s, double& distance, Wit& w
     const int a = lane == 3 ? 1 : 0;
     const int b = lane == 0 ? 1 : (lane == 1 ? 3 : (lane == 2 ? 2 : 3));
     const int c = lane == 0 ? 2 : (lane == 1 ? 1 : (lane == 2 ? 3 : 2));
-    make_face_fixed(F, 0, sh, a, b, c, centroid);
+    put_face(F, 0, build_face(sh, a, b, c, centroid));
   }
 
   const double tol = kEps64 * 1e2;
   int nverts = 4;
   int iteration = 0;
+  bool wrote = false;  // w, distance and normal hold a face's result
   while (iteration < 64 && nverts < kEpaMaxVerts - 1) {
     iteration++;
     F.valid &= ~F.pend;  // the reference's per-iteration recompute retires them
@@ -373,6 +401,7 @@ OGJK_DEV void epa(const BT& A, const BT& B, Simplex& s, double& distance, Wit& w
     const double improvement = dot(n, nv) - cdist;
     if (improvement < tol) {
       face_result(A, B, sh, cverts, n, cdist, distance, w, normal);
+      wrote = true;
       break;
     }
 
@@ -386,6 +415,7 @@ OGJK_DEV void epa(const BT& A, const BT& B, Simplex& s, double& distance, Wit& w
     }
     if (group_any<LPP>(dup)) {
       face_result(A, B, sh, cverts, n, cdist, distance, w, normal);
+      wrote = true;
       break;
     }
 
@@ -425,9 +455,9 @@ OGJK_DEV void epa(const BT& A, const BT& B, Simplex& s, double& distance, Wit& w
         const int pv = F.verts[q];
         const int x0 = pv & 0xff, x1 = (pv >> 8) & 0xff, x2 = (pv >> 16) & 0xff;
         const int e = 3 * vrank[q];
-        sh.edir[e + 0] = (unsigned short)(x0 | (x1 << 8));
-        sh.edir[e + 1] = (unsigned short)(x1 | (x2 << 8));
-        sh.edir[e + 2] = (unsigned short)(x2 | (x0 << 8));
+        sh.edir[e + 0] = edge_word(x0, x1);
+        sh.edir[e + 1] = edge_word(x1, x2);
+        sh.edir[e + 2] = edge_word(x2, x0);
       }
     }
     wave_sync();
@@ -438,15 +468,17 @@ OGJK_DEV void epa(const BT& A, const BT& B, Simplex& s, double& distance, Wit& w
     for (int base = 0; base < E; base += LPP) {
       const int e = base + lane;
       bool keep = false;
+      unsigned mine = 0u;
       if (e < E) {
-        const int key = undirected(sh.edir[e]);
+        mine = sh.edir[e];
+        const unsigned key = mine >> 16;
         int c = 0;
 #pragma unroll 8
-        for (int f = 0; f < E; f++) c += (undirected(sh.edir[f]) == key);
+        for (int f = 0; f < E; f++) c += (sh.edir[f] >> 16) == key;
         keep = (c == 1);
       }
       const uint64_t gb = group_ballot<LPP>(keep);
-      if (keep) sh.surv[nsurv + group_prefix<LPP>(gb)] = sh.edir[e];
+      if (keep) sh.surv[nsurv + group_prefix<LPP>(gb)] = (unsigned short)(mine & 0xffffu);
       nsurv += __popcll(gb);
     }
     wave_sync();
@@ -461,7 +493,7 @@ OGJK_DEV void epa(const BT& A, const BT& B, Simplex& s, double& distance, Wit& w
     for (int q = 0; q < FPL; q++) {
       if (fr[q] && frank[q] < nsurv) {
         const int ed = sh.surv[frank[q]];
-        make_face_fixed(F, q, sh, ed & 0xff, (ed >> 8) & 0xff, nid, centroid);
+        put_face(F, q, build_face(sh, ed & 0xff, (ed >> 8) & 0xff, nid, centroid));
       }
     }
   }
@@ -476,7 +508,21 @@ OGJK_DEV void epa(const BT& A, const BT& B, Simplex& s, double& distance, Wit& w
       int cverts;
       fetch_face<LPP>(F, cf, n, cverts);
       face_result(A, B, sh, cverts, n, cdist, distance, w, normal);
+      wrote = true;
     }
+  }
+  // the starting tetrahedron (the simplex this call returns) and, when no
+  // face result was written, the witnesses EPA started from
+  s.n = 4;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    s.p[k] = lds_v(sh, k);
+    s.ia[k] = sh.via[k];
+    s.ib[k] = sh.vib[k];
+  }
+  if (!wrote) {
+    w.w0 = D3{sh.w_in[0], sh.w_in[1], sh.w_in[2]};
+    w.w1 = D3{sh.w_in[3], sh.w_in[4], sh.w_in[5]};
   }
 }
 

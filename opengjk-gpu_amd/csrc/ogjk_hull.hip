@@ -351,12 +351,13 @@ __global__ __launch_bounds__(256) void k_center(float* __restrict__ pts,
                                                 const int32_t* __restrict__ cloud,
                                                 int64_t n_points,
                                                 const float* __restrict__ com) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n_points) return;
-  const float* c = com + 3 * (int64_t)cloud[i];
-  pts[3 * i] -= c[0];
-  pts[3 * i + 1] -= c[1];
-  pts[3 * i + 2] -= c[2];
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n_points;
+       i += (int64_t)gridDim.x * 256) {
+    const float* c = com + 3 * (int64_t)cloud[i];
+    pts[3 * i] -= c[0];
+    pts[3 * i + 1] -= c[1];
+    pts[3 * i + 2] -= c[2];
+  }
 }
 
 ogjk_status hip_err(hipError_t e, const char* what) {
@@ -409,7 +410,8 @@ ogjk_status ogjk_center_points(float* d_points, const int32_t* d_point_cloud, in
   if (n_points < 0) return ogjk_fail(OGJK_ERR_INVALID, "n_points < 0");
   if (n_points == 0) return OGJK_OK;
   if (!d_points || !d_point_cloud || !d_com) return ogjk_fail(OGJK_ERR_INVALID, "NULL argument");
-  hipLaunchKernelGGL(k_center, dim3((unsigned)((n_points + 255) / 256)), dim3(256), 0,
+  const int64_t cb = (n_points + 255) / 256;
+  hipLaunchKernelGGL(k_center, dim3((unsigned)(cb < 65536 ? cb : 65536)), dim3(256), 0,
                      (hipStream_t)stream, d_points, d_point_cloud, n_points, d_com);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? OGJK_OK : hip_err(e, "centre launch");

@@ -29,6 +29,7 @@ struct SceneArgs {       // SoA float scene (or double for the plain path)
   const float* cbounds = nullptr;
   const float* csbounds = nullptr;  // supercluster bounds; null: the scene has none
   int32_t cluster_min = 0x7fffffff;
+  const double* xyz = nullptr;  // plain path: FP64 AoS pool instead of x/y/z
 };
 
 struct PolyArgs {                // gkPolytope descriptors, AoS coordinates
@@ -55,8 +56,10 @@ constexpr int OGJK_F64_REC = 24;
 // Launch shape.  Pairs are bucketed by max(nA, nB): bucket b holds sizes
 // <= max_verts[b] (the last bucket is open-ended) and runs GJK on
 // gjk_lanes[b] (8, 16, 32, 64) lanes per pair and EPA on epa_lanes[b] (16,
-// 32, 64).  fused != 0 runs GJK+EPA as one kernel per pair instead of the
-// GJK -> worklist -> persistent EPA pipeline (bucket 0's EPA lanes).
+// 32, 64).  fused 1 runs GJK+EPA as one kernel per pair instead of the
+// GJK -> worklist -> persistent EPA pipeline (bucket 0's EPA lanes); fused 2
+// runs the plain-FP64 path in the reference's shape (a GJK kernel then an EPA
+// kernel, one group per pair each; GJK lanes may be 1) and other paths as 0.
 constexpr int kMaxBuckets = 4;
 struct LaunchConfig {
   int nb = 1;
@@ -110,6 +113,7 @@ hipError_t build_clusters(const float* x, const float* y, const float* z, const 
                           int32_t* orig, int64_t* body_slot, float* bounds, int64_t n_slots,
                           float* sbounds, int64_t* body_super, int64_t n_supers,
                           int64_t* n_supers_used, hipStream_t st);
+hipError_t launch_pack_f64_aos(const float* aos, int64_t n_verts, double* xyz, hipStream_t st);
 hipError_t launch_pack_f64(const float* aos, int64_t n_verts, double* x, double* y, double* z,
                            hipStream_t st);
 hipError_t launch_pack(const float* aos, const int64_t* off, const int32_t* cnt, int n_bodies,

@@ -164,6 +164,43 @@ struct PolySrc {
   }
 };
 
+// Plain-FP64 bodies in one AoS pool [x0, y0, z0, x1, ...] (the reference's
+// gkPolytope coordinate layout at FP64, GJK/common.h:68-78); body b is
+// [off[b], off[b] + cnt[b]).  Comparison path only.
+struct Aos64Src {
+  const double* xyz;
+  const int64_t* off;
+  const int32_t* cnt;
+  int32_t nb;
+  const int32_t* pairs;
+  using BodyT = Body<double, 3>;
+  static constexpr int kGjkWaves = OGJK_GJK_WAVES;
+  static constexpr bool kStage = false;
+
+  OGJK_DEV int pair_size(int i) const {
+    const int a = pairs ? pairs[2 * i] : 2 * i, b = pairs ? pairs[2 * i + 1] : 2 * i + 1;
+    if (a < 0 || a >= nb || b < 0 || b >= nb) return 0;
+    return cnt[a] > cnt[b] ? cnt[a] : cnt[b];
+  }
+
+  template <int LPP>
+  OGJK_DEV bool load(int i, int lane, BodyT& A, BodyT& B) const {
+    const int a = pairs ? pairs[2 * i] : 2 * i, b = pairs ? pairs[2 * i + 1] : 2 * i + 1;
+    if (a < 0 || a >= nb || b < 0 || b >= nb) return false;
+    const double* pa = xyz + 3 * off[a];
+    const double* pb = xyz + 3 * off[b];
+    A = BodyT{pa, pa + 1, pa + 2, cnt[a], 0.0f};
+    B = BodyT{pb, pb + 1, pb + 2, cnt[b], 0.0f};
+    return true;
+  }
+};
+
+// Sources of the plain-FP64 comparison path (the only ones the reference's
+// split launch shape is built for).
+template <class Src>
+constexpr bool kPlainSrc = std::is_same_v<Src, Aos64Src> ||
+                           std::is_same_v<typename Src::BodyT, Body<double, 1>>;
+
 // ----------------------------------------------------------------- records
 
 OGJK_DEV void store_invalid(const OutArgs& o, int i, bool epa) {
@@ -622,29 +659,96 @@ __global__ void __launch_bounds__(256, OGJK_EPA_WAVES)
   }
 }
 
+// The reference's launch shape (GJK/gpu/openGJK.cu:2968-3004): a GJK kernel
+// over every pair, then an EPA kernel over every pair, one group of LPP lanes
+// per pair in each, non-persistent (LPP = 1 is one thread per pair).  The
+// FP64 state of a penetrating pair passes through jobs[pair]; separated
+// pairs are finished by the GJK kernel and their job is marked n = 0, so
+// their EPA group returns at once, as the reference's EPA kernel does.
+template <int LPP, bool EMIT, class Src>
+__global__ void __launch_bounds__(256) k_gjk_all(Src src, OutArgs out, int n, EpaJob* jobs) {
+  constexpr int GPB = 256 / LPP;
+  const int lane = threadIdx.x & (LPP - 1);
+  const int pair = blockIdx.x * GPB + threadIdx.x / LPP;
+  if (pair >= n) return;
+  typename Src::BodyT A, B;
+  const bool ok = src.template load<LPP>(pair, lane, A, B) && A.n > 0 && B.n > 0;
+  if (!ok) {
+    if (lane == 0) {
+      store_invalid(out, pair, EMIT);
+      if (EMIT) jobs[pair].n = 0;
+    }
+    return;
+  }
+  Simplex s = {};
+  Wit w = {};
+  const double dist = gjk<LPP>(A, B, s, w, lane);
+  if (lane != 0) return;
+  if (EMIT && !(dist > kEps64)) {
+    job_store(jobs + pair, pair, s, w, dist);
+  } else {
+    if (EMIT) jobs[pair].n = 0;
+    store(out, pair, s, w, dist, EMIT ? contact_normal(w.w0, w.w1) : D3{0.0, 0.0, 0.0}, EMIT);
+  }
+}
+
+template <int LPP, class Src>
+__global__ void __launch_bounds__(256) k_epa_all(Src src, OutArgs out, int n, const EpaJob* jobs) {
+  constexpr int GPB = 256 / LPP;
+  __shared__ EpaShared sh[GPB];
+  const int lane = threadIdx.x & (LPP - 1);
+  const int g = threadIdx.x / LPP;
+  const int pair = blockIdx.x * GPB + g;
+  if (pair >= n || jobs[pair].n == 0) return;
+  Simplex s = {};
+  Wit w = {};
+  double dist;
+  job_load(jobs + pair, s, w, dist);
+  typename Src::BodyT A, B;
+  src.template load<LPP>(pair, lane, A, B);  // validated by the GJK kernel
+  D3 normal{0.0, 0.0, 0.0};
+  epa<LPP>(A, B, s, dist, w, normal, sh[g], lane);
+  if (lane == 0) store(out, pair, s, w, dist, normal, true);
+}
+
 __global__ void __launch_bounds__(256) k_zero(unsigned* c) {
   for (unsigned i = threadIdx.x; i < kCounterBytes / 4; i += 256) c[i] = 0u;
+}
+
+// Element-wise kernels over pools of any size run grid-stride over a capped
+// grid: one dispatch covers at most 2^32 work-items.
+constexpr unsigned kStrideBlocks = 65536;
+
+static unsigned stride_blocks(int64_t n) {
+  const int64_t b = (n + 255) / 256;
+  return (unsigned)(b < (int64_t)kStrideBlocks ? b : (int64_t)kStrideBlocks);
 }
 
 // AoS pool -> SoA arrays, one thread per vertex.
 __global__ void __launch_bounds__(256) k_pack(const float* __restrict__ aos, int64_t nv,
                                               float* __restrict__ x, float* __restrict__ y,
                                               float* __restrict__ z) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nv) return;
-  x[i] = aos[3 * i + 0];
-  y[i] = aos[3 * i + 1];
-  z[i] = aos[3 * i + 2];
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < nv; i += (int64_t)gridDim.x * 256) {
+    x[i] = aos[3 * i + 0];
+    y[i] = aos[3 * i + 1];
+    z[i] = aos[3 * i + 2];
+  }
 }
 
 __global__ void __launch_bounds__(256) k_pack_f64(const float* __restrict__ aos, int64_t nv,
                                                   double* __restrict__ x, double* __restrict__ y,
                                                   double* __restrict__ z) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nv) return;
-  x[i] = (double)aos[3 * i + 0];
-  y[i] = (double)aos[3 * i + 1];
-  z[i] = (double)aos[3 * i + 2];
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < nv; i += (int64_t)gridDim.x * 256) {
+    x[i] = (double)aos[3 * i + 0];
+    y[i] = (double)aos[3 * i + 1];
+    z[i] = (double)aos[3 * i + 2];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_pack_f64_aos(const float* __restrict__ aos, int64_t n,
+                                                      double* __restrict__ xyz) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    xyz[i] = (double)aos[i];
 }
 
 // Per-body max L1 norm, one wave per body.
@@ -672,9 +776,12 @@ static bool lanes_ok(int l, bool epa) {
 }
 
 int set_config(const LaunchConfig& c) {
-  if (c.nb < 1 || c.nb > kMaxBuckets) return -1;
+  if (c.nb < 1 || c.nb > kMaxBuckets || c.fused < 0 || c.fused > 2) return -1;
+  if (c.fused == 2 && c.nb != 1) return -1;
   for (int b = 0; b < c.nb; b++) {
-    if (!lanes_ok(c.gjk_lanes[b], false) || !lanes_ok(c.epa_lanes[b], true)) return -1;
+    // one lane per pair: GJK of the split shape only
+    const bool one = c.fused == 2 && c.gjk_lanes[b] == 1;
+    if (!(one || lanes_ok(c.gjk_lanes[b], false)) || !lanes_ok(c.epa_lanes[b], true)) return -1;
     if (b > 0 && c.max_verts[b] <= c.max_verts[b - 1] && b < c.nb - 1) return -1;
   }
   g_cfg = c;
@@ -809,13 +916,60 @@ static hipError_t launch_bucketed(int mode, const Src& src, const OutArgs& o, in
   return hipSuccess;
 }
 
+// EL = 0: GJK only (one launch).
+template <int GL, int EL, class Src>
+static hipError_t launch_split_lanes(const Src& src, const OutArgs& o, int n, void* ws,
+                                     hipStream_t st) {
+  EpaJob* jobs = (EpaJob*)((char*)ws + ws_jobs_off(n));
+  hipLaunchKernelGGL((k_gjk_all<GL, (EL > 0), Src>),
+                     dim3((unsigned)((n + 256 / GL - 1) / (256 / GL))), dim3(256), 0, st, src, o,
+                     n, jobs);
+  if constexpr (EL > 0) {
+    hipLaunchKernelGGL((k_epa_all<EL, Src>), dim3((unsigned)((n + 256 / EL - 1) / (256 / EL))),
+                       dim3(256), 0, st, src, o, n, (const EpaJob*)jobs);
+  }
+  return hipGetLastError();
+}
+
+template <int GL, class Src>
+static hipError_t launch_split_epa(int el, const Src& src, const OutArgs& o, int n, void* ws,
+                                   hipStream_t st) {
+  switch (el) {
+    case 0: return launch_split_lanes<GL, 0>(src, o, n, ws, st);
+    case 16: return launch_split_lanes<GL, 16>(src, o, n, ws, st);
+    case 32: return launch_split_lanes<GL, 32>(src, o, n, ws, st);
+    default: return launch_split_lanes<GL, 64>(src, o, n, ws, st);
+  }
+}
+
+// GJK (mode 1) or GJK then EPA (mode 3) in the reference's launch shape
+// (plain-FP64 sources only).
+template <class Src>
+static hipError_t launch_split(int mode, const Src& src, const OutArgs& o, int n, void* ws,
+                               hipStream_t st) {
+  if constexpr (kPlainSrc<Src>) {
+    const int el = mode == 3 ? g_cfg.epa_lanes[0] : 0;
+    switch (g_cfg.gjk_lanes[0]) {
+      case 1: return launch_split_epa<1>(el, src, o, n, ws, st);
+      case 8: return launch_split_epa<8>(el, src, o, n, ws, st);
+      case 16: return launch_split_epa<16>(el, src, o, n, ws, st);
+      case 32: return launch_split_epa<32>(el, src, o, n, ws, st);
+      default: return launch_split_epa<64>(el, src, o, n, ws, st);
+    }
+  } else {
+    return hipErrorInvalidValue;
+  }
+}
+
 template <class Src>
 static hipError_t launch_any(int mode, const Src& src, const OutArgs& o, int n, void* ws,
                              hipStream_t st) {
   if (n <= 0) return hipSuccess;
   if (mode == 2) return pairs_by_lanes<2>(src, o, n, st);
   if (mode != 1 && mode != 3) return hipErrorInvalidValue;
-  if (mode == 3 && g_cfg.fused) return pairs_by_lanes<3>(src, o, n, st);
+  // the split shape is the comparison path's; elsewhere it runs the pipeline
+  if (g_cfg.fused == 2 && kPlainSrc<Src>) return launch_split(mode, src, o, n, ws, st);
+  if (mode == 3 && g_cfg.fused == 1) return pairs_by_lanes<3>(src, o, n, st);
   return launch_bucketed(mode, src, o, n, ws, st);
 }
 
@@ -840,6 +994,10 @@ hipError_t launch_scene(int mode, const SceneArgs& s, const OutArgs& o, int n, v
 
 hipError_t launch_scene_f64(int mode, const SceneArgs& s, const OutArgs& o, int n, void* ws,
                             hipStream_t st) {
+  if (s.xyz) {
+    const Aos64Src src{s.xyz, s.off, s.cnt, s.n_bodies, s.pairs};
+    return launch_any(mode, src, o, n, ws, st);
+  }
   const SceneSrc<double> src{(const double*)s.x, (const double*)s.y, (const double*)s.z, s.off,
                              s.cnt, nullptr, s.n_bodies, s.pairs, ClusterView{}, nullptr,
                              0x7fffffff};
@@ -862,16 +1020,23 @@ hipError_t launch_poly(int mode, const PolyArgs& p, const OutArgs& o, int n, voi
 hipError_t launch_pack_f64(const float* aos, int64_t n_verts, double* x, double* y, double* z,
                            hipStream_t st) {
   if (n_verts <= 0) return hipSuccess;
-  const unsigned blocks = (unsigned)((n_verts + 255) / 256);
-  hipLaunchKernelGGL(k_pack_f64, dim3(blocks), dim3(256), 0, st, aos, n_verts, x, y, z);
+  hipLaunchKernelGGL(k_pack_f64, dim3(stride_blocks(n_verts)), dim3(256), 0, st, aos, n_verts, x,
+                     y, z);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_f64_aos(const float* aos, int64_t n_verts, double* xyz, hipStream_t st) {
+  if (n_verts <= 0) return hipSuccess;
+  const int64_t n = 3 * n_verts;
+  hipLaunchKernelGGL(k_pack_f64_aos, dim3(stride_blocks(n)), dim3(256), 0, st, aos, n, xyz);
   return hipGetLastError();
 }
 
 hipError_t launch_pack(const float* aos, const int64_t* off, const int32_t* cnt, int n_bodies,
                        int64_t n_verts, float* x, float* y, float* z, float* r1, hipStream_t st) {
   if (n_verts > 0) {
-    const unsigned blocks = (unsigned)((n_verts + 255) / 256);
-    hipLaunchKernelGGL(k_pack, dim3(blocks), dim3(256), 0, st, aos, n_verts, x, y, z);
+    hipLaunchKernelGGL(k_pack, dim3(stride_blocks(n_verts)), dim3(256), 0, st, aos, n_verts, x, y,
+                       z);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

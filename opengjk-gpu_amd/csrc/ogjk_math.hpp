@@ -75,7 +75,16 @@ OGJK_DEV int xchg_i32(int v) {
   else if constexpr (M == 2) return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);
   else if constexpr (M == 4) return __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);
   else if constexpr (M == 8) return __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);
-  else return __shfl_xor(v, M, 64);
+  else if constexpr (M == 16) {
+    // v_permlane16_swap with both operands = v: result 0 holds the even
+    // 16-lane row's value, result 1 the odd row's, in both rows of a pair
+    const auto r = __builtin_amdgcn_permlane16_swap((unsigned)v, (unsigned)v, false, false);
+    return (int)((lane_id() & 16) ? r[0] : r[1]);
+  } else {
+    static_assert(M == 32, "exchange step");
+    const auto r = __builtin_amdgcn_permlane32_swap((unsigned)v, (unsigned)v, false, false);
+    return (int)((lane_id() & 32) ? r[0] : r[1]);
+  }
 }
 template <int M>
 OGJK_DEV float xchg(float v) {

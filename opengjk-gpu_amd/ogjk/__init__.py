@@ -46,7 +46,7 @@ C_ABI_SYMBOLS = [
     "ogjk_server_info", "ogjk_server_destroy", "ogjk_broad_phase_workspace", "ogjk_broad_phase",
     "ogjk_expand_pairs_workspace", "ogjk_expand_pairs", "ogjk_collide_scene_contacts",
     "ogjk_collide_posed_bodies_contacts", "ogjk_build_hulls", "ogjk_hull_moments",
-    "ogjk_center_points", "ogjk_collide_scene_noalloc",
+    "ogjk_center_points", "ogjk_collide_scene_noalloc", "ogjk_pack_scene_f64_aos",
 ]
 
 
@@ -79,7 +79,7 @@ class _Grid(C.Structure):
 
 class _Scene64(C.Structure):
     _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("body_off", C.c_void_p),
-                ("body_cnt", C.c_void_p), ("n_bodies", C.c_int32)]
+                ("body_cnt", C.c_void_p), ("n_bodies", C.c_int32), ("xyz", C.c_void_p)]
 
 
 _lib = None
@@ -105,6 +105,7 @@ def lib():
                                                 vp, C.c_size_t, vp]),
             "ogjk_pack_scene": (C.c_int, [vp, vp, vp, i32, i64, vp, vp, vp, vp, vp]),
             "ogjk_pack_scene_f64": (C.c_int, [vp, i64, vp, vp, vp, vp]),
+            "ogjk_pack_scene_f64_aos": (C.c_int, [vp, i64, vp, vp]),
             "ogjk_cluster_slots": (i64, [i64, i32]),
             "ogjk_cluster_supers": (i64, [i64]),
             "ogjk_graph_capture_scene": (C.c_int, [C.c_int, C.POINTER(_Scene), vp, i32, vp, vp, vp,
@@ -166,9 +167,14 @@ def _ptr(a):
     return None if a is None else a.ctypes.data
 
 
-def set_config(gjk_lanes: int = 16, epa_lanes: int = 32, fused: bool = False) -> None:
-    """Launch shape (see ogjk_set_config in include/opengjk/ogjk.h)."""
-    _check(lib().ogjk_set_config(int(gjk_lanes), int(epa_lanes), int(bool(fused))))
+PIPELINE, FUSED, SPLIT = 0, 1, 2  # launch shapes of ogjk_set_config
+
+
+def set_config(gjk_lanes: int = 16, epa_lanes: int = 32, fused: int = PIPELINE) -> None:
+    """Launch shape (see ogjk_set_config in include/opengjk/ogjk.h): fused is
+    PIPELINE (0), FUSED (1) or SPLIT (2, the plain-FP64 path in the
+    reference's two-launch shape)."""
+    _check(lib().ogjk_set_config(int(gjk_lanes), int(epa_lanes), int(fused)))
 
 
 def set_buckets(max_verts, gjk_lanes, epa_lanes) -> None:
@@ -224,7 +230,9 @@ def collide_host_indexed(mode: int, coords, off, cnt, pairs, simplices=None, dis
 # ------------------------------------------------------------ device scenes
 
 class DeviceScene:
-    """A batch resident in device memory as an SoA scene (torch tensors)."""
+    """A batch resident in device memory as an SoA scene (torch tensors).
+    f64: also keep the plain-FP64 comparison scene, SoA (True), one AoS pool
+    ("aos") or both ("both")."""
 
     def __init__(self, batch, device="cuda", f64: bool = False, stream=None,
                  clusters: bool = False, cluster_min: int = 512):
@@ -245,15 +253,22 @@ class DeviceScene:
         _check(lib().ogjk_pack_scene(aos.data_ptr(), self.off.data_ptr(), self.cnt.data_ptr(),
                                      self.n_bodies, self.n_verts, self.x.data_ptr(),
                                      self.y.data_ptr(), self.z.data_ptr(), self.r1.data_ptr(), st))
-        self.f64 = None
-        if f64:
+        self.f64 = self.f64_aos = None
+        if f64 in ("aos", "both"):
+            # plain FP64 in the reference's gkPolytope layout: one AoS pool
+            self.xyz64 = torch.empty(3 * nv, dtype=torch.float64, device=dev)
+            _check(lib().ogjk_pack_scene_f64_aos(aos.data_ptr(), self.n_verts,
+                                                 self.xyz64.data_ptr(), st))
+            self.f64_aos = _Scene64(None, None, None, self.off.data_ptr(), self.cnt.data_ptr(),
+                                    self.n_bodies, self.xyz64.data_ptr())
+        if f64 and f64 != "aos":
             self.x64 = torch.empty(nv, dtype=torch.float64, device=dev)
             self.y64 = torch.empty(nv, dtype=torch.float64, device=dev)
             self.z64 = torch.empty(nv, dtype=torch.float64, device=dev)
             _check(lib().ogjk_pack_scene_f64(aos.data_ptr(), self.n_verts, self.x64.data_ptr(),
                                              self.y64.data_ptr(), self.z64.data_ptr(), st))
             self.f64 = _Scene64(self.x64.data_ptr(), self.y64.data_ptr(), self.z64.data_ptr(),
-                                self.off.data_ptr(), self.cnt.data_ptr(), self.n_bodies)
+                                self.off.data_ptr(), self.cnt.data_ptr(), self.n_bodies, None)
         torch.cuda.synchronize()
         del aos
         self.scene = _Scene(self.x.data_ptr(), self.y.data_ptr(), self.z.data_ptr(),
@@ -393,17 +408,20 @@ class DeviceScene:
             "f64": t.zeros((n, F64_RECORD), dtype=t.float64, device=dev) if f64 else None,
         }
 
-    def collide(self, mode: int, n_pairs: int, pairs=None, out=None, plain_f64: bool = False,
+    def collide(self, mode: int, n_pairs: int, pairs=None, out=None, plain_f64=False,
                 f64_record: bool = False, stream=None):
-        """Launch on `stream` (default: torch's current stream); asynchronous."""
+        """Launch on `stream` (default: torch's current stream); asynchronous.
+        plain_f64: run the plain-FP64 comparison path over the SoA (True) or
+        AoS ("aos") FP64 copy."""
         out = out or self.outputs(n_pairs, f64_record)
         pp = None if pairs is None else pairs.data_ptr()
         f = out["f64"].data_ptr() if out.get("f64") is not None else None
         st = self._stream(stream)
         if plain_f64:
-            if self.f64 is None:
-                raise OgjkError("scene built without f64=True")
-            _check(lib().ogjk_collide_scene_f64(mode, C.byref(self.f64), pp, n_pairs,
+            sc64 = self.f64_aos if plain_f64 == "aos" else self.f64
+            if sc64 is None:
+                raise OgjkError(f"scene built without its plain-FP64 copy ({plain_f64!r})")
+            _check(lib().ogjk_collide_scene_f64(mode, C.byref(sc64), pp, n_pairs,
                                                 out["simplices"].data_ptr(),
                                                 out["distances"].data_ptr(),
                                                 out["normals"].data_ptr(), f, st))

@@ -151,6 +151,42 @@ def test_plain_f64_equals_mixed(torch_cuda):
     assert np.array_equal(simp_m.view(np.uint8), simp_p.view(np.uint8))
 
 
+# The plain-FP64 comparison kernels in every launch shape the benchmark
+# times: fused (one kernel), split (the reference's GJK kernel then EPA
+# kernel, GJK/gpu/openGJK.cu:2968-3004) over SoA and AoS FP64 pools, and one
+# thread per pair for GJK.  Each must equal the FP64 oracle bit for bit.
+PLAIN_SHAPES = [(1, 16, 16, True), (1, 64, 64, True), (2, 8, 32, "aos"), (2, 8, 32, True),
+                (2, 1, 32, True), (2, 16, 64, True), (2, 32, 16, "aos")]
+
+
+@pytest.mark.parametrize("shape,gl,el,layout", PLAIN_SHAPES)
+@pytest.mark.parametrize("mode", [oracle.GJK, oracle.GJK_EPA])
+def test_plain_f64_shapes(torch_cuda, shape, gl, el, layout, mode):
+    if mode == oracle.GJK and shape == 1:
+        pytest.skip("fused applies to GJK+EPA")
+    b = gen.make(gen.ELLIPSOID, 1500, seed=41, box=1.5, nmin=4, nmax=700)
+    ref = oracle_run(mode, b)
+    sc = ogjk.DeviceScene(b, f64="both")
+    ogjk.set_config(gl, el, shape)
+    try:
+        out = sc.collide(mode, b.n_pairs, f64_record=True, plain_f64=layout)
+        torch_cuda.cuda.synchronize()
+    finally:
+        ogjk.set_config(8, 32, 0)
+    rec = out["f64"].cpu().numpy()
+    simp = ogjk.simplices_from_device(out["simplices"])
+    check(compare_f64(rec, simp, ref, mode), f"plain shape {shape} {gl}/{el} {layout} mode{mode}")
+    r32 = compare_f32(simp, out["distances"].cpu().numpy(), out["normals"].cpu().numpy(), ref, mode)
+    assert r32["exact"] == r32["n"], r32
+
+
+def test_split_shape_rejects_one_lane_elsewhere(torch_cuda):
+    """One GJK lane per pair exists only in the split shape."""
+    with pytest.raises(ogjk.OgjkError):
+        ogjk.set_config(1, 32, 0)
+    ogjk.set_config(8, 32, 0)
+
+
 def test_known_answers_dropin(torch_cuda):
     b = cases.known_answer_batch()
     simp, dist, nrm = ogjk.collide_host(oracle.GJK_EPA, b)
