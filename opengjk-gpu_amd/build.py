@@ -67,6 +67,18 @@ def build_lib() -> str:
     return out
 
 
+def build_tests() -> str:
+    """The test driver of the reference-named C++ API (tests/dropin_api_check.hip),
+    linked against the in-tree libogjk.so."""
+    src = os.path.join(ROOT, "tests", "dropin_api_check.hip")
+    bindir = os.path.join(ROOT, "tests", "bin")
+    os.makedirs(bindir, exist_ok=True)
+    out = os.path.join(bindir, "dropin_api_check")
+    _run([HIPCC, *DEVICE_FLAGS, f"-I{INC}", "-o", out, src, f"-L{LIBDIR}", "-logjk",
+          "-Wl,-rpath,$ORIGIN/../../opengjk-gpu_amd/lib"])
+    return out
+
+
 def build_oracle() -> None:
     """The checker: oracle restatement always; the reference build only where
     the reference checkout exists (never on the GPU host)."""
@@ -80,9 +92,11 @@ def build_all() -> None:
     build_gen()
     build_oracle()
     build_lib()
+    build_tests()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "gen": build_gen, "lib": build_lib, "oracle": build_oracle}[w]()
+        {"all": build_all, "gen": build_gen, "lib": build_lib, "oracle": build_oracle,
+         "tests": build_tests}[w]()
