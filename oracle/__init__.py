@@ -36,7 +36,8 @@ def simplex_dtype(bits: int) -> np.dtype:
 
 
 STATS_DTYPE = np.dtype([("gjk_iters", np.int32), ("epa_iters", np.int32),
-                        ("sweeps_a", np.int32), ("sweeps_b", np.int32)])
+                        ("sweeps_a", np.int32), ("sweeps_b", np.int32),
+                        ("face_overflow", np.int32)])
 
 _libs: dict = {}
 

@@ -1142,7 +1142,10 @@ void oracle_epa(const opoly* bd1, const opoly* bd2, osimplex* simplex, oreal* di
           break;
         }
       }
-      if (slot < 0 || slot >= EPA_MAX_FACES) break;
+      if (slot < 0 || slot >= EPA_MAX_FACES) {
+        if (st) st->face_overflow++; /* horizon edges left without a face */
+        break;
+      }
       oface* F = &P.faces[slot];
       F->v[0] = edges[i].v1;
       F->v[1] = edges[i].v2;

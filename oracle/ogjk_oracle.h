@@ -42,13 +42,15 @@ typedef struct {
 } osimplex;
 
 /* Per-pair instrumentation (not part of the reference): number of GJK
- * iterations, EPA main-loop iterations, and body sweeps done by the support
- * functions for body 1 and body 2. */
+ * iterations, EPA main-loop iterations, body sweeps done by the support
+ * functions for body 1 and body 2, and EPA iterations that ran out of face
+ * slots with horizon edges left (GJK/cpu/EPA.c:766-775). */
 typedef struct {
   int gjk_iters;
   int epa_iters;
   int sweeps_a;
   int sweeps_b;
+  int face_overflow;
 } ostats;
 
 #ifdef __cplusplus

@@ -123,3 +123,99 @@ def lattice_batch(n: int = 200, seed: int = 7):
         A.append(bodies[0])
         B.append(bodies[1])
     return gen.from_bodies(A, B)
+
+
+# ------------------------------------------------ reference EPA test shapes
+#
+# The shapes of the reference's EPA tests (examples/gpu/example.cu:122-262,
+# cases 5-8 at :593-892).  The reference draws its random points from C
+# rand(); here a seeded numpy generator draws the same kind of point set,
+# computed in float32 like the reference's gkFloat arithmetic.
+
+def ref_polytope(n: int, offset, seed: int):
+    """generatePolytope (example.cu:122-137): n points at radius
+    U[1, 1.5] in spherical coordinates theta U[0, 2 pi], phi U[0, pi]."""
+    f = np.float32
+    rng = np.random.default_rng(seed)
+    pi = f(3.14159265358979323846)
+    u = rng.random((n, 3)).astype(np.float32)
+    theta = u[:, 0] * f(2.0) * pi
+    phi = u[:, 1] * pi
+    r = f(1.0) + u[:, 2] * f(0.5)
+    sp = np.sin(phi.astype(np.float64)).astype(np.float32)
+    p = np.stack([r * sp * np.cos(theta.astype(np.float64)).astype(np.float32),
+                  r * sp * np.sin(theta.astype(np.float64)).astype(np.float32),
+                  r * np.cos(phi.astype(np.float64)).astype(np.float32)], 1)
+    return (p + np.asarray(offset, np.float32)).astype(np.float32)
+
+
+def grid_cube(grid: int = 40, size: float = 1.0, offset=(0.0, 0.0, 0.0)):
+    """generateCubeWithGrid (example.cu:145-227): grid x grid points on each
+    of the six faces, in the reference's face and loop order, float32."""
+    f = np.float32
+    sz = f(size)
+    t = np.array([-sz + (f(2.0) * sz * f(i)) / f(grid - 1) for i in range(grid)], np.float32)
+    a, b = np.meshgrid(t, t, indexing="ij")
+    a, b = a.reshape(-1), b.reshape(-1)
+    full = np.full_like(a, sz)
+    faces = [(full, a, b), (-full, a, b), (a, full, b), (a, -full, b), (a, b, full), (a, b, -full)]
+    p = np.concatenate([np.stack(fc, 1) for fc in faces]).astype(np.float32)
+    return (p + np.asarray(offset, np.float32)).astype(np.float32)
+
+
+def rotate45_xyz_shift_x(p, shift: float = 1.0):
+    """Rotate by 45 degrees about x, then y, then z, and move +shift in x, in
+    float32 as example.cu:663-697 (and EPAUsage.cpp:55-80) does."""
+    f = np.float32
+    ang = f(f(45.0) * f(math.pi) / f(180.0))
+    ca, sa = f(math.cos(float(ang))), f(math.sin(float(ang)))
+    px, py, pz = (p[:, k].astype(np.float32) for k in range(3))
+    py, pz = py * ca - pz * sa, py * sa + pz * ca
+    px, pz = px * ca + pz * sa, -px * sa + pz * ca
+    px, py = px * ca - py * sa, px * sa + py * ca
+    return np.stack([px + f(shift), py, pz], 1).astype(np.float32)
+
+
+def sphere_surface(n: int, radius: float, offset, seed: int):
+    """generateSphereSurface (example.cu:233-254): theta = 2 pi u,
+    phi = acos(2 v - 1), points at the given radius."""
+    f = np.float32
+    rng = np.random.default_rng(seed)
+    u = rng.random(n).astype(np.float32)
+    v = rng.random(n).astype(np.float32)
+    pi = f(3.14159265358979323846)
+    theta = f(2.0) * pi * u
+    phi = np.arccos((f(2.0) * v - f(1.0)).astype(np.float64)).astype(np.float32)
+    r = f(radius)
+    sp = np.sin(phi.astype(np.float64)).astype(np.float32)
+    p = np.stack([r * sp * np.cos(theta.astype(np.float64)).astype(np.float32),
+                  r * sp * np.sin(theta.astype(np.float64)).astype(np.float32),
+                  r * np.cos(phi.astype(np.float64)).astype(np.float32)], 1)
+    return (p + np.asarray(offset, np.float32)).astype(np.float32)
+
+
+def reference_epa_cases():
+    """Cases 5-7 of the reference's EPA tests as one batch: 50-point
+    polytopes 0.5 apart (case 5), 40x40-grid cubes with the second rotated
+    45 degrees about every axis and moved +1 in x (case 6, 9,600 vertices
+    each), 1,000-point spheres of radius 2 with centres 1 apart (case 7;
+    case 8 is the same pair through separate GJK and EPA calls)."""
+    A = [ref_polytope(50, (0, 0, 0), 51), grid_cube(), sphere_surface(1000, 2.0, (0, 0, 0), 71)]
+    B = [ref_polytope(50, (0.5, 0, 0), 52), rotate45_xyz_shift_x(grid_cube()),
+         sphere_surface(1000, 2.0, (1.0, 0, 0), 72)]
+    return gen.from_bodies(A, B)
+
+
+def epa_cap_pairs(n_search: int = 200_000):
+    """Pairs of the uneven workload (4-4096 vertices, seed 2) that the FP64
+    oracle's EPA runs to its 64-iteration cap, most of them also running out
+    of its 128 face slots (GJK/cpu/EPA.c:766-775, 828-863).  Returns the
+    batch of those pairs."""
+    import oracle
+    b = gen.make(gen.ELLIPSOID, n_search, seed=2, box=1.5, nmin=4, nmax=4096)
+    r = oracle.run(oracle.GJK_EPA, b.coords, b.off, b.cnt, b.pa, b.pb, bits=64, nthreads=8,
+                   stats=True)
+    pick = np.where(r.stats["epa_iters"] >= 64)[0]
+    A = [b.body(int(b.pa[i])) for i in pick]
+    B = [b.body(int(b.pb[i])) for i in pick]
+    return gen.from_bodies(A, B)

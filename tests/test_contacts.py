@@ -1,6 +1,8 @@
 """Contact records (ogjk_collide_scene_contacts, ogjk_collide_posed_bodies_contacts):
-each 40-byte record must equal the gkSimplex witnesses, distance and normal
-that the ABI outputs of the same launch hold, bit for bit."""
+each 40-byte record must equal the FP64 oracle's distance, contact normal and
+witnesses rounded once to float32 (the fields collision_response_kernel reads,
+visualization/integrate_final_gjk.cu:572-689), and the gkSimplex witnesses,
+distance and normal that the ABI outputs of the same launch hold, bit for bit."""
 import numpy as np
 import pytest
 
@@ -50,6 +52,64 @@ def test_scene_contacts_equal_abi_outputs(torch_cuda, mode):
     assert same_bits(got, ref)
     assert np.array_equal(con["f64"][:n].cpu().numpy().view(np.uint64),
                           out["f64"][:n].cpu().numpy().view(np.uint64))
+
+
+def oracle_records(ref, mode):
+    """The FP64 oracle's results as contact records, rounded once to float32:
+    distance, normal (zero without EPA), witness on A, witness on B."""
+    n = len(ref.distances)
+    nrm = ref.normals if mode & oracle.EPA else np.zeros((n, 3))
+    w = ref.simplices["witnesses"].reshape(n, 6)
+    return np.concatenate([ref.distances[:, None], nrm, w], 1).astype(np.float32)
+
+
+@pytest.mark.parametrize("mode", [oracle.GJK, oracle.GJK_EPA])
+def test_scene_contacts_equal_oracle(torch_cuda, mode):
+    """Uneven sizes (buckets and clusters) and tie-heavy bodies."""
+    import cases
+    torch = torch_cuda
+    for name, b in (("uneven", gen.make(gen.ELLIPSOID, 3000, seed=67, box=1.5, nmin=4, nmax=4096)),
+                    ("degenerate", cases.degenerate_batch(2000, seed=68))):
+        ref = oracle.run(mode, b.coords, b.off, b.cnt, b.pa, b.pb, bits=64, nthreads=8)
+        sc = ogjk.DeviceScene(b)
+        sc.build_clusters(256)
+        ogjk.set_buckets([128], [8, 64], [32, 64])
+        try:
+            con = sc.collide_contacts(mode, b.n_pairs)
+            torch.cuda.synchronize()
+        finally:
+            ogjk.set_config(8, 32, False)
+        got = con["contacts"][:b.n_pairs].cpu().numpy()
+        want = oracle_records(ref, mode)
+        same = (got == want) | (np.isnan(got) & np.isnan(want))
+        print(f"\n[contacts vs oracle, {name}, mode {mode}] {b.n_pairs} pairs, "
+              f"{(want[:, 0] <= 0).mean():.2f} penetrating", flush=True)
+        assert same.all(), f"{name}: records differ at pairs {np.where(~same.all(1))[0][:10]}"
+
+
+def test_scene_step_contacts_equal_oracle(torch_cuda):
+    """Posed bodies from the broad phase: records against the oracle run on
+    the FP64 world vertices of each pair."""
+    torch = torch_cuda
+    b = gen.make(gen.ELLIPSOID, 1000, seed=69, box=0.0, nmin=8, nmax=512)
+    nb = len(b.cnt)
+    radii = gen.body_radii(b.coords, b.off, b.cnt)
+    poses_h = gen.body_poses(nb, seed=70, tbox=12.0)
+    grid = ogjk.Grid.for_spheres(float(radii.max()), 12.0, 30)
+    sc = ogjk.DeviceScene(b)
+    step = ogjk.SceneStep(sc, radii, grid, max_pairs=20 * nb, contacts=True)
+    pairs, found, con = step(torch.from_numpy(poses_h).cuda())
+    torch.cuda.synchronize()
+    pr = pairs.cpu().numpy()
+    n = len(pr)
+    side = np.concatenate([poses_h[pr[:, 0]], poses_h[pr[:, 1]]], 1)
+    pool, off, cnt, pa, pb = oracle.posed_pool(b.coords, b.off, b.cnt, pr, side)
+    ref = oracle.run(oracle.GJK_EPA, pool, off, cnt, pa, pb, bits=64, nthreads=8)
+    got = con["contacts"][:n].cpu().numpy()
+    want = oracle_records(ref, oracle.GJK_EPA)
+    print(f"\n[scene step contacts vs oracle] {n} pairs, {(want[:, 0] <= 0).mean():.2f} "
+          "penetrating", flush=True)
+    assert n > nb // 2 and np.array_equal(got.view(np.uint32), want.view(np.uint32))
 
 
 def test_scene_contacts_invalid_pairs_are_nan(torch_cuda):
