@@ -260,6 +260,12 @@ ogjk_status ogjk_server_wait(ogjk_server server, int64_t ticket, double* latency
 ogjk_status ogjk_server_poll(ogjk_server server, int64_t ticket, int* done);
 ogjk_status ogjk_server_info(ogjk_server server, int32_t* slots, int32_t* capacity,
                              int64_t* launches, int64_t* pairs_done);
+/* Shuts the server down and frees it: submissions are refused from then on,
+ * requests not yet launched are cancelled (their ogjk_server_wait returns
+ * OGJK_ERR_INVALID), launched batches complete and their requests are
+ * answered, and the call returns once every thread blocked in
+ * ogjk_server_wait has returned.  No call may start on the server after
+ * destroy has been called.  Returns OGJK_ERR_HIP if a launch had failed. */
 ogjk_status ogjk_server_destroy(ogjk_server server);
 
 /* ------------------------------------------------------------- broad phase */

@@ -66,6 +66,15 @@ __global__ __launch_bounds__(kBlock) void k_bp_hist(const float4* __restrict__ s
   slot[i] = atomicAdd(count + k, 1);
 }
 
+// Crowded cells: objects sorted by (cell, index) by a stable radix sort
+// instead of ranked by a scan of their cell.  order[t] = the object at
+// sorted position t, okeys[t] its cell.
+__global__ __launch_bounds__(kBlock) void k_bp_iota(int32_t* __restrict__ v, int n) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) v[i] = i;
+}
+
+
 __global__ __launch_bounds__(kBlock) void k_bp_place(const uint32_t* __restrict__ keys,
                                                      const int32_t* __restrict__ slot,
                                                      const int32_t* __restrict__ start, int n,
@@ -82,18 +91,35 @@ struct alignas(32) BpRec {
   int32_t pad[2];
 };
 
+// A cell holding more objects than this is ordered by the radix sort (the
+// per-object rank scan costs its cell's occupancy).
+constexpr int kRankScanMax = 64;
+
+// The count and emit kernels of the rank-scan pass run unless *gate is set
+// (some cell was too crowded to rank), and always when gate is null (the
+// radix-sorted pass).
+__device__ __forceinline__ bool gated_off(const int32_t* gate) {
+  return gate != nullptr && *gate != 0;
+}
+
 // Object i goes to start[cell] + (number of lower indices in its cell): cells
 // list their objects in ascending index.  pos[i] = that position.
+// An object of a cell holding more than kRankScanMax objects ranks nothing
+// and sets *crowded (one atomic per wave); the host then sorts instead.
 __global__ __launch_bounds__(kBlock) void k_bp_rank(const float4* __restrict__ sph,
                                                     const uint32_t* __restrict__ keys,
                                                     const int32_t* __restrict__ start,
                                                     const int32_t* __restrict__ tmp, int n,
                                                     BpRec* __restrict__ rec,
-                                                    int32_t* __restrict__ pos) {
+                                                    int32_t* __restrict__ pos,
+                                                    int32_t* __restrict__ crowded) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t k = keys[i];
-  const int lo = start[k], hi = start[k + 1];
+  const uint32_t k = i < n ? keys[i] : 0u;
+  const int lo = i < n ? start[k] : 0, hi = i < n ? start[k + 1] : 0;
+  const bool big = hi - lo > kRankScanMax;
+  const uint64_t any = __ballot(big);
+  if (any != 0 && (threadIdx.x & 63) == (unsigned)__builtin_ctzll(any)) atomicOr(crowded, 1);
+  if (i >= n || big) return;
   int r = 0;
   for (int t = lo; t < hi; t++) r += tmp[t] < i;
   BpRec e;
@@ -103,6 +129,23 @@ __global__ __launch_bounds__(kBlock) void k_bp_rank(const float4* __restrict__ s
   e.pad[0] = e.pad[1] = 0;
   rec[lo + r] = e;
   pos[i] = lo + r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bp_from_sorted(const float4* __restrict__ sph,
+                                                           const uint32_t* __restrict__ okeys,
+                                                           const int32_t* __restrict__ order,
+                                                           int n, BpRec* __restrict__ rec,
+                                                           int32_t* __restrict__ pos) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n) return;
+  const int i = order[t];
+  BpRec e;
+  e.s = sph[i];
+  e.id = i;
+  e.key = okeys[t];
+  e.pad[0] = e.pad[1] = 0;
+  rec[t] = e;
+  pos[i] = t;
 }
 
 // Partners of an object the count pass keeps for the emit pass.
@@ -196,8 +239,10 @@ __global__ __launch_bounds__(kBlock) void k_bp_count(const BpRec* __restrict__ r
                                                      const int32_t* __restrict__ start, int n,
                                                      int g, int cap,
                                                      int64_t* __restrict__ counts,
-                                                     int32_t* __restrict__ keep) {
+                                                     int32_t* __restrict__ keep,
+                                                     const int32_t* __restrict__ gate) {
   const int s = xcd_block() * kBlock + threadIdx.x;
+  if (gated_off(gate)) return;
   if (s >= n) {
     if (s == n) counts[n] = 0;
     return;
@@ -223,9 +268,10 @@ __global__ __launch_bounds__(kBlock) void k_bp_emit(const BpRec* __restrict__ re
                                                     const int32_t* __restrict__ keep,
                                                     const int64_t* __restrict__ offsets,
                                                     gkCollisionPair* __restrict__ pairs,
-                                                    int64_t max_pairs) {
+                                                    int64_t max_pairs,
+                                                    const int32_t* __restrict__ gate) {
   const int o = blockIdx.x * kBlock + threadIdx.x;
-  if (o >= n) return;
+  if (o >= n || gated_off(gate)) return;
   int64_t w = offsets[o];
   const int64_t c = counts[o];
   if (c <= kKeep) {
@@ -279,23 +325,51 @@ inline size_t up256(size_t b) { return (b + 255) & ~size_t(255); }
 
 // Workspace of the broad phase for n objects over `cells` cells.
 struct BpLayout {
-  size_t keys, slot, count, start, tmp, rec, pos, keep, counts, offsets, temp, total;
+  size_t keys, slot, count, start, tmp, rec, pos, keep, counts, offsets, okeys, temp, total;
   size_t temp_bytes;
+  int key_bits;  // bits of a cell key (the radix sort's range)
 };
 
+hipError_t bp_layout_query(int n, int64_t cells, BpLayout* L);
+
+// The library's temp-storage queries cost host time on every call, so the
+// layout of the last (n, cells) is kept per thread.
 hipError_t bp_layout(int n, int64_t cells, BpLayout* L) {
-  size_t cell_b = 0, scan_b = 0;
+  thread_local int last_n = -1;
+  thread_local int64_t last_cells = -1;
+  thread_local BpLayout last;
+  if (n != last_n || cells != last_cells) {
+    const hipError_t e = bp_layout_query(n, cells, &last);
+    if (e != hipSuccess) return e;
+    last_n = n;
+    last_cells = cells;
+  }
+  *L = last;
+  return hipSuccess;
+}
+
+hipError_t bp_layout_query(int n, int64_t cells, BpLayout* L) {
+  size_t cell_b = 0, scan_b = 0, sort_b = 0;
   hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, cell_b, (const int32_t*)nullptr,
                                                   (int32_t*)nullptr, (int)(cells + 1));
   if (e != hipSuccess) return e;
   e = hipcub::DeviceScan::ExclusiveSum(nullptr, scan_b, (const int64_t*)nullptr,
                                        (int64_t*)nullptr, n + 1);
   if (e != hipSuccess) return e;
+  L->key_bits = 1;
+  while (L->key_bits < 31 && ((int64_t)1 << L->key_bits) < cells) L->key_bits++;
+  e = hipcub::DeviceRadixSort::SortPairs(nullptr, sort_b, (const uint32_t*)nullptr,
+                                         (uint32_t*)nullptr, (const int32_t*)nullptr,
+                                         (int32_t*)nullptr, n, 0, L->key_bits);
+  if (e != hipSuccess) return e;
   L->temp_bytes = cell_b > scan_b ? cell_b : scan_b;
+  if (sort_b > L->temp_bytes) L->temp_bytes = sort_b;
   size_t o = 0;
   L->keys = o;    o += up256(sizeof(uint32_t) * (size_t)n);
   L->slot = o;    o += up256(sizeof(int32_t) * (size_t)n);
-  L->count = o;   o += up256(sizeof(int32_t) * ((size_t)cells + 1));
+  // count[cells] stays 0 (the scan's last input), count[cells + 1] = the
+  // crowded-cell flag
+  L->count = o;   o += up256(sizeof(int32_t) * ((size_t)cells + 2));
   L->start = o;   o += up256(sizeof(int32_t) * ((size_t)cells + 1));
   L->tmp = o;     o += up256(sizeof(int32_t) * (size_t)n);
   L->rec = o;     o += up256(sizeof(BpRec) * (size_t)n);
@@ -303,6 +377,7 @@ hipError_t bp_layout(int n, int64_t cells, BpLayout* L) {
   L->keep = o;    o += up256(sizeof(int32_t) * kKeep * (size_t)n);
   L->counts = o;  o += up256(sizeof(int64_t) * ((size_t)n + 1));
   L->offsets = o; o += up256(sizeof(int64_t) * ((size_t)n + 1));
+  L->okeys = o;   o += up256(sizeof(uint32_t) * (size_t)n);
   L->temp = o;    o += up256(L->temp_bytes);
   L->total = o;
   return hipSuccess;
@@ -403,35 +478,61 @@ ogjk_status ogjk_broad_phase(const float* d_spheres, int32_t n_objects, const og
   const float4* sph = (const float4*)d_spheres;
 
   // counting sort by cell; start[] is the exclusive scan of the histogram
-  const int64_t zb = blocks_for(cells + 1);
+  const int64_t zb = blocks_for(cells + 2);
+  int32_t* crowded = count + cells + 1;
   hipLaunchKernelGGL(k_bp_zero, dim3((unsigned)(zb < 4096 ? zb : 4096)), dim3(kBlock), 0, st_,
-                     count, cells + 1);
+                     count, cells + 2);
   hipLaunchKernelGGL(k_bp_hist, dim3(blocks_for(n)), dim3(kBlock), 0, st_, sph, n,
                      grid->cell_size, grid->boundary, g, keys, slot, count);
   BP_HIP(hipGetLastError(), "cell histogram");
   size_t tb = L.temp_bytes;
   BP_HIP(hipcub::DeviceScan::ExclusiveSum(temp, tb, count, start, (int)(cells + 1), st_),
          "cell scan");
+  // Cells of at most kRankScanMax objects: each object ranks itself within its
+  // cell.  When a cell holds more, k_bp_rank sets *crowded, the count and
+  // emit kernels of this pass return at once, and the host then orders the
+  // objects with the radix sort below and counts and emits again.
   hipLaunchKernelGGL(k_bp_place, dim3(blocks_for(n)), dim3(kBlock), 0, st_, keys, slot, start,
                      n, tmp);
   hipLaunchKernelGGL(k_bp_rank, dim3(blocks_for(n)), dim3(kBlock), 0, st_, sph, keys, start, tmp,
-                     n, rec, pos);
+                     n, rec, pos, crowded);
   BP_HIP(hipGetLastError(), "cell sort");
-  hipLaunchKernelGGL(k_bp_count, dim3(xcd_blocks((int64_t)n + 1)), dim3(kBlock), 0, st_, rec,
-                     start, n, g, grid->max_per_cell, counts, keep);
-  BP_HIP(hipGetLastError(), "pair count");
-  tb = L.temp_bytes;
-  BP_HIP(hipcub::DeviceScan::ExclusiveSum(temp, tb, counts, offsets, n + 1, st_), "pair scan");
-  if (max_pairs > 0) {
-    hipLaunchKernelGGL(k_bp_emit, dim3(blocks_for(n)), dim3(kBlock), 0, st_, rec, start, pos, n,
-                       g, grid->max_per_cell, counts, keep, offsets, d_pairs,
-                       (int64_t)max_pairs);
-    BP_HIP(hipGetLastError(), "pair emit");
-  }
   int64_t total = 0;
-  BP_HIP(hipMemcpyAsync(&total, offsets + n, sizeof(int64_t), hipMemcpyDeviceToHost, st_),
-         "pair total");
-  BP_HIP(hipStreamSynchronize(st_), "broad phase");
+  int32_t is_crowded = 0;
+  for (int pass = 0; pass < 2; pass++) {
+    const int32_t* gate = pass == 0 ? crowded : nullptr;
+    hipLaunchKernelGGL(k_bp_count, dim3(xcd_blocks((int64_t)n + 1)), dim3(kBlock), 0, st_, rec,
+                       start, n, g, grid->max_per_cell, counts, keep, gate);
+    BP_HIP(hipGetLastError(), "pair count");
+    tb = L.temp_bytes;
+    BP_HIP(hipcub::DeviceScan::ExclusiveSum(temp, tb, counts, offsets, n + 1, st_), "pair scan");
+    if (max_pairs > 0) {
+      hipLaunchKernelGGL(k_bp_emit, dim3(blocks_for(n)), dim3(kBlock), 0, st_, rec, start, pos,
+                         n, g, grid->max_per_cell, counts, keep, offsets, d_pairs,
+                         (int64_t)max_pairs, gate);
+      BP_HIP(hipGetLastError(), "pair emit");
+    }
+    BP_HIP(hipMemcpyAsync(&total, offsets + n, sizeof(int64_t), hipMemcpyDeviceToHost, st_),
+           "pair total");
+    if (pass == 0)
+      BP_HIP(hipMemcpyAsync(&is_crowded, crowded, sizeof(int32_t), hipMemcpyDeviceToHost, st_),
+             "cell occupancy");
+    BP_HIP(hipStreamSynchronize(st_), "broad phase");
+    if (!is_crowded) break;
+    if (pass == 0) {
+      // crowded cells: a stable radix sort by cell lists each cell's objects
+      // in ascending index
+      auto* okeys = (uint32_t*)(ws + L.okeys);
+      hipLaunchKernelGGL(k_bp_iota, dim3(blocks_for(n)), dim3(kBlock), 0, st_, slot, n);
+      tb = L.temp_bytes;
+      BP_HIP(hipcub::DeviceRadixSort::SortPairs(temp, tb, keys, okeys, slot, tmp, n, 0,
+                                                L.key_bits, st_),
+             "cell radix sort");
+      hipLaunchKernelGGL(k_bp_from_sorted, dim3(blocks_for(n)), dim3(kBlock), 0, st_, sph, okeys,
+                         tmp, n, rec, pos);
+      BP_HIP(hipGetLastError(), "cell sort");
+    }
+  }
   if (n_found) *n_found = total;
   if (n_pairs) *n_pairs = (int32_t)(total < max_pairs ? total : max_pairs);
   return OGJK_OK;

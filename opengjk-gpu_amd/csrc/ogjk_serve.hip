@@ -15,6 +15,11 @@
 //    results on the host) and wakes its waiter.
 // Each pair is computed by the same kernels and lane layout whatever batch it
 // lands in, so a request's results do not depend on the mix.
+//
+// Shutdown (ogjk_server_destroy): new submissions are refused, requests not
+// yet launched are cancelled (their wait returns an error), launched batches
+// complete and are answered as usual, and the server is freed only after
+// every thread blocked in ogjk_server_wait has returned.
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -51,6 +56,7 @@ struct Request {
   double latency_s = 0.0;
   int64_t batch = -1;  // launch sequence number that carried it
   bool done = false;
+  bool cancelled = false;  // never launched: the server was destroyed first
   ogjk_status status = OGJK_OK;
 };
 
@@ -88,12 +94,40 @@ struct ogjk_server_s {
   std::deque<std::shared_ptr<Request>> pending;
   std::unordered_map<int64_t, std::shared_ptr<Request>> tickets;
   std::thread dispatcher, completer;
-  bool stop = false;
+  bool stop = false;     // threads exit
+  bool closing = false;  // destroy has begun: no new submissions
+  int waiters = 0;       // threads inside ogjk_server_wait
   int64_t next_ticket = 1;
   int64_t launches = 0;
   int64_t pairs_done = 0;
   int device = 0;
   hipError_t async_error = hipSuccess;
+
+  // Cancel what is not launched, let launched batches complete, wait for
+  // every waiter to leave, then stop the threads.
+  void shutdown() {
+    {
+      std::unique_lock<std::mutex> lock(mu);
+      closing = true;
+      for (auto& r : pending) {
+        r->status = OGJK_ERR_INVALID;
+        r->cancelled = true;
+        r->done = true;
+      }
+      pending.clear();
+      cv_done.notify_all();
+      cv_done.wait(lock, [&] {
+        for (const Slot& sl : slots)
+          if (sl.busy) return false;
+        return waiters == 0;
+      });
+      stop = true;
+    }
+    cv_dispatch.notify_all();
+    cv_complete.notify_all();
+    if (dispatcher.joinable()) dispatcher.join();
+    if (completer.joinable()) completer.join();
+  }
 
   ~ogjk_server_s() {
     {
@@ -348,6 +382,7 @@ ogjk_status ogjk_server_submit(ogjk_server s, const int32_t* pairs, const float*
   r->t_submit = Clock::now();
   {
     std::lock_guard<std::mutex> lock(s->mu);
+    if (s->closing) return ogjk_fail(OGJK_ERR_INVALID, "server is being destroyed");
     r->ticket = s->next_ticket++;
     s->tickets[r->ticket] = r;
     s->pending.push_back(r);
@@ -366,11 +401,17 @@ ogjk_status ogjk_server_wait(ogjk_server s, int64_t ticket, double* latency_s, i
     if (it == s->tickets.end()) return ogjk_fail(OGJK_ERR_INVALID, "unknown ticket %lld",
                                                  (long long)ticket);
     r = it->second;
+    s->waiters++;
     s->cv_done.wait(lock, [&] { return r->done; });
-    s->tickets.erase(it);
+    s->waiters--;
+    s->tickets.erase(ticket);
+    // a destroy in progress waits for the last waiter to leave
+    if (s->closing && s->waiters == 0) s->cv_done.notify_all();
   }
   if (latency_s) *latency_s = r->latency_s;
   if (batch) *batch = r->batch;
+  if (r->cancelled)
+    return ogjk_fail(r->status, "request cancelled: the server was destroyed before launching it");
   if (r->status != OGJK_OK) return ogjk_fail(r->status, "request failed on the device");
   return OGJK_OK;
 }
@@ -396,7 +437,11 @@ ogjk_status ogjk_server_info(ogjk_server s, int32_t* slots, int32_t* capacity, i
 }
 
 ogjk_status ogjk_server_destroy(ogjk_server s) {
+  if (!s) return OGJK_OK;
+  s->shutdown();
+  const hipError_t e = s->async_error;
   delete s;
+  if (e != hipSuccess) return ogjk_fail(OGJK_ERR_HIP, "server: %s", hipGetErrorString(e));
   return OGJK_OK;
 }
 

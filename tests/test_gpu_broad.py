@@ -46,6 +46,9 @@ SCENES = {
     "fine_128": dict(n=100_000, box=40.0, rmax=0.3, outside=0.02, max_grid=128, cap=0),
     "one_cell": dict(n=3_000, box=3.0, rmax=0.2, outside=0.0, max_grid=1, cap=0),
     "cap_one": dict(n=5_000, box=2.0, rmax=0.3, outside=0.0, max_grid=8, cap=1),
+    # crowded cells (more than 64 objects) are ordered by the radix-sort path
+    "crowded_cell_20k": dict(n=20_000, box=3.0, rmax=0.1, outside=0.0, max_grid=1, cap=0),
+    "crowded_capped": dict(n=20_000, box=3.0, rmax=0.1, outside=0.3, max_grid=2, cap=512),
 }
 
 

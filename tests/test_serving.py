@@ -116,3 +116,54 @@ def test_alone_vs_mix(torch_cuda, scene):
         assert r["exact"] == r["n"], (k, r["inexact"])
         assert np.array_equal(out["distances"][:m], ref.distances.astype(np.float32))
         assert np.array_equal(out["normals"][:m], ref.normals.astype(np.float32))
+
+
+def test_destroy_with_outstanding_requests(torch_cuda, scene):
+    """ogjk_server_destroy while requests are pending and threads are blocked
+    in ogjk_server_wait: every waiter returns (answered, or cancelled when
+    its request was never launched), destroy returns, answered requests hold
+    the same results as the device entry point, and later submissions to a
+    new server work."""
+    import time
+    torch = torch_cuda
+    b, sc = scene
+    reqs = gen.posed_requests(len(b.cnt), 60, seed=45, kmin=1500, kmax=1500, tbox=1.5)
+    srv = ogjk.Server(sc, max_batch_pairs=2048, slots=2)
+    tickets = [srv.submit(p, q) for p, q in reqs]
+    got = [None] * len(tickets)
+
+    def waiter(k):
+        try:
+            out, _, _ = srv.wait(tickets[k])
+            got[k] = ("ok", out["distances"].copy())
+        except ogjk.OgjkError as e:
+            got[k] = ("err", str(e))
+
+    threads = [threading.Thread(target=waiter, args=(k,)) for k in range(len(tickets))]
+    for t in threads:
+        t.start()
+    time.sleep(0.5)  # every waiter is inside its (single) wait call
+    done = threading.Event()
+    closer = threading.Thread(target=lambda: (srv.close(), done.set()))
+    closer.start()
+    closer.join(timeout=60)
+    assert done.is_set(), "destroy did not return"
+    for t in threads:
+        t.join(timeout=60)
+    assert all(g is not None for g in got)
+    ok = [k for k, g in enumerate(got) if g[0] == "ok"]
+    cancelled = [k for k, g in enumerate(got) if g[0] == "err"]
+    print(f"\n[destroy] {len(ok)} answered, {len(cancelled)} cancelled", flush=True)
+    assert all("cancelled" in got[k][1] for k in cancelled)
+    assert len(ok) >= 1
+    for k in ok[:10]:
+        p, q = reqs[k]
+        dev = sc.collide_posed(oracle.GJK_EPA, torch.from_numpy(p).cuda(), torch.from_numpy(q).cuda())
+        torch.cuda.synchronize()
+        assert np.array_equal(got[k][1].view(np.uint32), dev["distances"].cpu().numpy().view(np.uint32))
+    srv2 = ogjk.Server(sc, max_batch_pairs=2048, slots=2)
+    try:
+        out, _, _ = srv2.wait(srv2.submit(*reqs[0]))
+        assert np.isfinite(out["distances"]).all()
+    finally:
+        srv2.close()
