@@ -265,7 +265,7 @@ def batch_leg(name: str, args, headline: bool) -> dict:
     if not (args.lanes and headline) and os.path.isfile(tpath):
         table = autotune.load(tpath)
         if not (headline and args.cluster_min is not None):
-            cmin = table["cluster_min"]
+            cmin = table["cluster_min"]  # None: the table sweeps every body whole
     ogjk.set_config(lg, le, PIPELINE)
     wl = dict(spec["gen"])
     if args.pairs and headline:
@@ -311,6 +311,8 @@ def batch_leg(name: str, args, headline: bool) -> dict:
                    "mode": "GJK then EPA" if mode == 3 else "GJK only",
                    "lanes_per_pair": ({"buckets_max_verts": table["max_verts"],
                                        "gjk": table["gjk_lanes"], "epa": table["epa_lanes"],
+                                       "gjk_waves_per_block": table.get("gjk_waves"),
+                                       "epa_waves_per_block": table.get("epa_waves"),
                                        "table": "opengjk-gpu_amd/tuning/" + spec["tuning"]}
                                       if table else {"gjk": lg, "epa": le}),
                    "clusters": {"min_verts": cmin, "build_s": build_s} if cmin else None,

@@ -123,6 +123,14 @@ ogjk_status ogjk_set_config(int gjk_lanes, int epa_lanes, int fused);
 ogjk_status ogjk_set_buckets(int n_buckets, const int* max_verts, const int* gjk_lanes,
                              const int* epa_lanes);
 
+/* Pairs per workgroup of the size buckets set by ogjk_set_config or
+ * ogjk_set_buckets (which reset them to GJK 1 and EPA 4 waves per block):
+ * waves per block of the GJK kernel and of the persistent EPA kernel (1, 2
+ * or 4) for each of the n_buckets buckets; pairs per block = 64 * waves /
+ * lanes per pair.  EPA runs one launch per lane class, shaped by the first
+ * bucket of the class.  Same process-wide scope as ogjk_set_config. */
+ogjk_status ogjk_set_block_waves(int n_buckets, const int* gjk_waves, const int* epa_waves);
+
 /* Device workspace bytes a GJK or GJK+EPA call over n_pairs needs. */
 size_t ogjk_workspace_bytes(int32_t n_pairs);
 

@@ -267,6 +267,19 @@ ogjk_status ogjk_set_buckets(int n_buckets, const int* max_verts, const int* gjk
   return OGJK_OK;
 }
 
+ogjk_status ogjk_set_block_waves(int n_buckets, const int* gjk_waves, const int* epa_waves) {
+  ogjk::LaunchConfig c = ogjk::get_config();
+  if (n_buckets != c.nb || !gjk_waves || !epa_waves)
+    return fail(OGJK_ERR_INVALID, "%d bucket(s) configured, got %d", c.nb, n_buckets);
+  for (int b = 0; b < n_buckets; b++) {
+    c.gjk_waves[b] = gjk_waves[b];
+    c.epa_waves[b] = epa_waves[b];
+  }
+  if (ogjk::set_config(c) != 0)
+    return fail(OGJK_ERR_INVALID, "waves per block must be 1, 2 or 4");
+  return OGJK_OK;
+}
+
 size_t ogjk_workspace_bytes(int32_t n_pairs) { return ogjk::workspace_bytes(n_pairs); }
 
 ogjk_status ogjk_sync(void* stream) {

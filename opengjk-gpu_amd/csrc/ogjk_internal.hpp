@@ -66,6 +66,10 @@ struct LaunchConfig {
   int max_verts[kMaxBuckets] = {0x7fffffff, 0x7fffffff, 0x7fffffff, 0x7fffffff};
   int gjk_lanes[kMaxBuckets] = {8, 8, 8, 8};
   int epa_lanes[kMaxBuckets] = {32, 32, 32, 32};
+  // waves per block (1, 2, 4) of the GJK kernel and of the persistent EPA
+  // kernel (a class's launch takes the first bucket of the class)
+  int gjk_waves[kMaxBuckets] = {1, 1, 1, 1};
+  int epa_waves[kMaxBuckets] = {4, 4, 4, 4};
   int fused = 0;
 };
 

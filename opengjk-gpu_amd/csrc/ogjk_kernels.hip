@@ -24,6 +24,10 @@
 
 namespace ogjk {
 
+// Dynamic LDS of the GJK staging area and of the persistent EPA kernel, sized
+// per launch for the block's groups (waves per block are a launch parameter).
+extern __shared__ __attribute__((aligned(16))) unsigned char ogjk_lds[];
+
 // ----------------------------------------------------------------- sources
 
 // CL: 0 no clusters (the kernels contain no culled sweep at all), 1 clusters,
@@ -590,10 +594,10 @@ __global__ void __launch_bounds__(256) k_scatter(Src src, int n, BucketPlan plan
 // its EPA class's job list with its FP64 state.  With a single bucket the
 // list is the identity and is not materialised.
 template <int LPP, bool EMIT, class Src>
-__global__ void __launch_bounds__(64, Src::kGjkWaves)
+__global__ void __launch_bounds__(256, Src::kGjkWaves)
     k_gjk(Src src, OutArgs out, int n, BucketPlan plan, int bucket, unsigned* c, const int* list,
           EpaJob* jobs) {
-  constexpr int GPB = 64 / LPP;
+  const int GPB = (int)blockDim.x / LPP;  // pairs per block: 1, 2 or 4 waves of groups
   const int lane = threadIdx.x & (LPP - 1);
   // the partition's results are read-only here (plain, cached loads)
   const bool multi = plan.nb > 1;
@@ -610,8 +614,8 @@ __global__ void __launch_bounds__(64, Src::kGjkWaves)
     return;
   }
   if constexpr (Src::kStage) {
-    __shared__ float stg[GPB][3 * kStageGjk];
-    stage_pair<LPP, kStageGjk>(A, B, stg[threadIdx.x / LPP], lane);
+    float* stg = (float*)ogjk_lds + (threadIdx.x / LPP) * (3 * kStageGjk);
+    stage_pair<LPP, kStageGjk>(A, B, stg, lane);
   }
   Simplex s = {};
   Wit w = {};
@@ -633,8 +637,8 @@ __global__ void __launch_bounds__(64, Src::kGjkWaves)
 template <int LPP, class Src>
 __global__ void __launch_bounds__(256, OGJK_EPA_WAVES)
     k_epa_list(Src src, OutArgs out, BucketPlan plan, int cls, unsigned* c, const EpaJob* jobs) {
-  constexpr int GPB = 256 / LPP;
-  __shared__ EpaShared sh[GPB];
+  const int GPB = (int)blockDim.x / LPP;  // groups per block: 1, 2 or 4 waves of them
+  EpaShared* sh = (EpaShared*)ogjk_lds;
   const int lane = threadIdx.x & (LPP - 1);
   const int g = threadIdx.x / LPP;
   const unsigned njobs =
@@ -775,10 +779,13 @@ static bool lanes_ok(int l, bool epa) {
   return (!epa && l == 8) || l == 16 || l == 32 || l == 64;
 }
 
+static bool waves_ok(int w) { return w == 1 || w == 2 || w == 4; }
+
 int set_config(const LaunchConfig& c) {
   if (c.nb < 1 || c.nb > kMaxBuckets || c.fused < 0 || c.fused > 2) return -1;
   if (c.fused == 2 && c.nb != 1) return -1;
   for (int b = 0; b < c.nb; b++) {
+    if (!waves_ok(c.gjk_waves[b]) || !waves_ok(c.epa_waves[b])) return -1;
     // one lane per pair: GJK of the split shape only
     const bool one = c.fused == 2 && c.gjk_lanes[b] == 1;
     if (!(one || lanes_ok(c.gjk_lanes[b], false)) || !lanes_ok(c.epa_lanes[b], true)) return -1;
@@ -833,36 +840,43 @@ static unsigned persistent_blocks(int n, int gpb, unsigned cap = 4096u) {
   return want < cap ? want : cap;
 }
 
+// wpb: waves per block (1, 2, 4).  The grid covers every pair (blocks past
+// the bucket's count return at once).
 template <int LPP, bool EMIT, class Src>
 static hipError_t launch_gjk(const Src& src, const OutArgs& o, int n, const BucketPlan& p, int b,
-                             void* ws, hipStream_t st) {
+                             int wpb, void* ws, hipStream_t st) {
   unsigned* c = (unsigned*)ws;
   const int* list = (const int*)((char*)ws + ws_list_off());
   EpaJob* jobs = (EpaJob*)((char*)ws + ws_jobs_off(n));
-  constexpr int GPB = 64 / LPP;
-  hipLaunchKernelGGL((k_gjk<LPP, EMIT, Src>), dim3((unsigned)((n + GPB - 1) / GPB)), dim3(64), 0,
-                     st, src, o, n, p, b, c, list, jobs);
+  const int gpb = 64 * wpb / LPP;
+  const size_t lds = Src::kStage ? (size_t)gpb * 3 * kStageGjk * sizeof(float) : 0;
+  hipLaunchKernelGGL((k_gjk<LPP, EMIT, Src>), dim3((unsigned)((n + gpb - 1) / gpb)),
+                     dim3(64 * wpb), lds, st, src, o, n, p, b, c, list, jobs);
   return hipGetLastError();
 }
 
+// Persistent: at most 16,384 waves (four per SIMD of the chip), whatever the
+// block size.
 template <int LPP, class Src>
 static hipError_t launch_epa_list(const Src& src, const OutArgs& o, int n, const BucketPlan& p,
-                                  int cls, void* ws, hipStream_t st) {
+                                  int cls, int wpb, void* ws, hipStream_t st) {
   unsigned* c = (unsigned*)ws;
   const EpaJob* jobs = (const EpaJob*)((char*)ws + ws_jobs_off(n));
-  hipLaunchKernelGGL((k_epa_list<LPP, Src>), dim3(persistent_blocks(n, 256 / LPP)), dim3(256), 0,
-                     st, src, o, p, cls, c, jobs);
+  const int gpb = 64 * wpb / LPP;
+  hipLaunchKernelGGL((k_epa_list<LPP, Src>), dim3(persistent_blocks(n, gpb, 16384u / wpb)),
+                     dim3(64 * wpb), (size_t)gpb * sizeof(EpaShared), st, src, o, p, cls, c,
+                     jobs);
   return hipGetLastError();
 }
 
 template <bool EMIT, class Src>
 static hipError_t gjk_by_lanes(int lanes, const Src& src, const OutArgs& o, int n,
-                               const BucketPlan& p, int b, void* ws, hipStream_t st) {
+                               const BucketPlan& p, int b, int wpb, void* ws, hipStream_t st) {
   switch (lanes) {
-    case 8: return launch_gjk<8, EMIT>(src, o, n, p, b, ws, st);
-    case 16: return launch_gjk<16, EMIT>(src, o, n, p, b, ws, st);
-    case 32: return launch_gjk<32, EMIT>(src, o, n, p, b, ws, st);
-    default: return launch_gjk<64, EMIT>(src, o, n, p, b, ws, st);
+    case 8: return launch_gjk<8, EMIT>(src, o, n, p, b, wpb, ws, st);
+    case 16: return launch_gjk<16, EMIT>(src, o, n, p, b, wpb, ws, st);
+    case 32: return launch_gjk<32, EMIT>(src, o, n, p, b, wpb, ws, st);
+    default: return launch_gjk<64, EMIT>(src, o, n, p, b, wpb, ws, st);
   }
 }
 
@@ -896,8 +910,9 @@ static hipError_t launch_bucketed(int mode, const Src& src, const OutArgs& o, in
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   for (int b = 0; b < p.nb; b++) {
-    e = mode == 3 ? gjk_by_lanes<true>(cfg.gjk_lanes[b], src, o, n, p, b, ws, st)
-                  : gjk_by_lanes<false>(cfg.gjk_lanes[b], src, o, n, p, b, ws, st);
+    const int w = cfg.gjk_waves[b];
+    e = mode == 3 ? gjk_by_lanes<true>(cfg.gjk_lanes[b], src, o, n, p, b, w, ws, st)
+                  : gjk_by_lanes<false>(cfg.gjk_lanes[b], src, o, n, p, b, w, ws, st);
     if (e != hipSuccess) return e;
   }
   if (mode != 3) return hipSuccess;
@@ -906,10 +921,12 @@ static hipError_t launch_bucketed(int mode, const Src& src, const OutArgs& o, in
     const int cls = p.epa_class[b];
     if (done[cls]) continue;
     done[cls] = true;
+    // one launch per EPA class, shaped by the first bucket of the class
+    const int w = cfg.epa_waves[b];
     switch (cfg.epa_lanes[b]) {
-      case 16: e = launch_epa_list<16>(src, o, n, p, cls, ws, st); break;
-      case 32: e = launch_epa_list<32>(src, o, n, p, cls, ws, st); break;
-      default: e = launch_epa_list<64>(src, o, n, p, cls, ws, st); break;
+      case 16: e = launch_epa_list<16>(src, o, n, p, cls, w, ws, st); break;
+      case 32: e = launch_epa_list<32>(src, o, n, p, cls, w, ws, st); break;
+      default: e = launch_epa_list<64>(src, o, n, p, cls, w, ws, st); break;
     }
     if (e != hipSuccess) return e;
   }

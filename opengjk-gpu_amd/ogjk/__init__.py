@@ -47,6 +47,7 @@ C_ABI_SYMBOLS = [
     "ogjk_expand_pairs_workspace", "ogjk_expand_pairs", "ogjk_collide_scene_contacts",
     "ogjk_collide_posed_bodies_contacts", "ogjk_build_hulls", "ogjk_hull_moments",
     "ogjk_center_points", "ogjk_collide_scene_noalloc", "ogjk_pack_scene_f64_aos",
+    "ogjk_set_block_waves",
 ]
 
 
@@ -100,6 +101,7 @@ def lib():
             "ogjk_sync": (C.c_int, [vp]),
             "ogjk_set_config": (C.c_int, [C.c_int, C.c_int, C.c_int]),
             "ogjk_set_buckets": (C.c_int, [C.c_int, vp, vp, vp]),
+            "ogjk_set_block_waves": (C.c_int, [C.c_int, vp, vp]),
             "ogjk_workspace_bytes": (C.c_size_t, [i32]),
             "ogjk_collide_scene_ws": (C.c_int, [C.c_int, C.POINTER(_Scene), vp, i32, vp, vp, vp, vp,
                                                 vp, C.c_size_t, vp]),
@@ -184,6 +186,14 @@ def set_buckets(max_verts, gjk_lanes, epa_lanes) -> None:
     gl = np.array(gjk_lanes, dtype=np.int32)
     el = np.array(epa_lanes, dtype=np.int32)
     _check(lib().ogjk_set_buckets(nb, _ptr(mv), _ptr(gl), _ptr(el)))
+
+
+def set_block_waves(gjk_waves, epa_waves) -> None:
+    """Waves per block of the GJK and EPA kernels per size bucket (see
+    ogjk_set_block_waves in include/opengjk/ogjk.h)."""
+    gw = np.array(gjk_waves, dtype=np.int32)
+    ew = np.array(epa_waves, dtype=np.int32)
+    _check(lib().ogjk_set_block_waves(len(gw), _ptr(gw), _ptr(ew)))
 
 
 def device_count() -> int:
@@ -390,8 +400,14 @@ class DeviceScene:
         return Graph(self, mode, n_pairs, pairs, out), out
 
     def drop_clusters(self) -> None:
-        """Sweep every body without culling."""
+        """Sweep every body without culling (the clusters stay built)."""
         self.scene.clusters = None
+
+    def use_clusters(self) -> None:
+        """Cull again through the clusters built by build_clusters."""
+        if self.clusters is None:
+            raise OgjkError("no clusters built")
+        self.scene.clusters = C.pointer(self.clusters)
 
     def _stream(self, stream):
         if stream is None:

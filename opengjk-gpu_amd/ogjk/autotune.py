@@ -4,12 +4,16 @@ bodies are swept through their clusters.  Every candidate is timed on the
 device with HIP events over whole GJK+EPA launches; the best is written to a
 JSON table that `apply()` loads.
 
-Search (coordinate descent, a few dozen launches):
-  1. one bucket: lanes (GJK, EPA) in LANES_1, cluster threshold in CLUSTER_MIN;
+Search (coordinate descent, about seventy launches):
+  1. one bucket: lanes (GJK, EPA) in LANES_1, cluster threshold in CLUSTER_MIN
+     or no clusters at all;
   2. two buckets: a small-body bucket split off at each of SPLITS with lanes
      in SMALL_LANES, the large bucket keeping the step-1 choice;
   3. three buckets: the best two-bucket table with a middle bucket split off
-     at each larger boundary, lanes in MID_LANES.
+     at each larger boundary, lanes in MID_LANES;
+  4. the cluster threshold again, on the best table;
+  5. pairs per workgroup: for each bucket, the GJK kernel's and then the EPA
+     kernel's waves per block in WAVES (pairs per block = 64 * waves / lanes).
 """
 from __future__ import annotations
 
@@ -19,7 +23,8 @@ import time
 import ogjk
 
 LANES_1 = [(64, 64), (32, 64), (32, 32), (16, 32)]
-CLUSTER_MIN = [128, 256, 512, 1024]
+CLUSTER_MIN = [128, 256, 512, 1024, None]  # None: sweep every body whole
+WAVES = [1, 2, 4]
 SPLITS = [16, 32, 64, 128, 256]
 SMALL_LANES = [(8, 16), (8, 32), (16, 32)]
 MID_LANES = [(16, 32), (32, 32), (32, 64)]
@@ -44,13 +49,25 @@ def _apply_table(t):
         ogjk.set_config(t["gjk_lanes"][0], t["epa_lanes"][0], False)
     else:
         ogjk.set_buckets(t["max_verts"], t["gjk_lanes"], t["epa_lanes"])
+    nb = len(t["gjk_lanes"])
+    if "gjk_waves" in t or "epa_waves" in t:
+        ogjk.set_block_waves(t.get("gjk_waves", [1] * nb), t.get("epa_waves", [4] * nb))
+
+
+def _set_clusters(scene, cmin) -> None:
+    if cmin is None:
+        scene.drop_clusters()
+    else:
+        scene.use_clusters()
+        scene.set_cluster_min(cmin)
 
 
 def apply(table: dict, scene=None) -> None:
-    """Make `table` the launch shape (and the scene's cluster threshold)."""
+    """Make `table` the launch shape (and the scene's cluster threshold; a
+    null threshold sweeps every body whole)."""
     _apply_table(table)
-    if scene is not None and table.get("cluster_min") and scene.clusters is not None:
-        scene.set_cluster_min(table["cluster_min"])
+    if scene is not None and scene.clusters is not None:
+        _set_clusters(scene, table.get("cluster_min"))
 
 
 def load(path: str) -> dict:
@@ -67,11 +84,12 @@ def tune(scene, n_pairs: int, log=print) -> dict:
 
     def run(table, cmin):
         _apply_table(table)
-        scene.set_cluster_min(cmin)
+        _set_clusters(scene, cmin)
         ms = _time(scene, n_pairs)
         tried.append(dict(table, cluster_min=cmin, ms=ms))
         log(f"  {table['max_verts']} gjk {table['gjk_lanes']} epa {table['epa_lanes']} "
-            f"clusters>={cmin}: {ms:.3f} ms")
+            f"waves {table.get('gjk_waves')} {table.get('epa_waves')} clusters>={cmin}: "
+            f"{ms:.3f} ms")
         return ms
 
     best, best_ms = None, float("inf")
@@ -102,6 +120,17 @@ def tune(scene, n_pairs: int, log=print) -> dict:
             ms = run(best, cmin)
             if ms < best_ms:
                 best_ms, best_c = ms, cmin
+    nb = len(best["gjk_lanes"])
+    best = dict(best, gjk_waves=[1] * nb, epa_waves=[4] * nb)
+    for b in range(nb):
+        for key in ("gjk_waves", "epa_waves"):
+            for w in WAVES:
+                if w == best[key][b]:
+                    continue
+                t = dict(best, **{key: best[key][:b] + [w] + best[key][b + 1:]})
+                ms = run(t, best_c)
+                if ms < best_ms:
+                    best, best_ms = t, ms
     result = dict(best, cluster_min=best_c, ms=best_ms, pairs=n_pairs,
                   candidates=len(tried), search_s=time.perf_counter() - t0)
     apply(result, scene)
@@ -125,6 +154,9 @@ def main() -> int:
                  nmax=args.nmax, nthreads=min(len(os.sched_getaffinity(0)), 16))
     sc = ogjk.DeviceScene(b)
     sc.build_clusters(CLUSTER_MIN[0])
+    t = time.perf_counter()
+    _time(sc, b.n_pairs, reps=1)  # warm-up
+    print(f"# warm-up {time.perf_counter() - t:.2f}s", flush=True)
     res = tune(sc, b.n_pairs, log=lambda m: print(m, flush=True))
     res["workload"] = {"family": args.family, "pairs": args.pairs, "seed": args.seed,
                        "box": args.box, "nmin": args.nmin, "nmax": args.nmax}

@@ -143,6 +143,22 @@ def test_bucketed_parity(torch_cuda, table, cmin):
     check(compare_f64(rec, simp, ref, oracle.GJK_EPA), f"buckets {table} clusters {cmin}")
 
 
+# Pairs per workgroup: GJK and EPA blocks of 1, 2 or 4 waves per bucket.
+@pytest.mark.parametrize("gw,ew", [([2, 4], [1, 2]), ([4, 1], [2, 1]), ([1, 2], [4, 4])])
+def test_block_waves_parity(torch_cuda, gw, ew):
+    b = gen.make(gen.ELLIPSOID, 3000, seed=32, box=1.5, nmin=4, nmax=2048)
+    ref = oracle_run(oracle.GJK_EPA, b)
+    ogjk.set_buckets([96], [8, 32], [32, 64])
+    ogjk.set_block_waves(gw, ew)
+    try:
+        rec, simp, _, _ = run_scene(torch_cuda, b, oracle.GJK_EPA, cluster_min=512)
+    finally:
+        ogjk.set_config(8, 32, False)
+    check(compare_f64(rec, simp, ref, oracle.GJK_EPA), f"block waves gjk {gw} epa {ew}")
+    with pytest.raises(ogjk.OgjkError):
+        ogjk.set_block_waves([3], [1])
+
+
 def test_plain_f64_equals_mixed(torch_cuda):
     b = gen.make(gen.SPHERE, 2048, seed=11, nverts=64, box=1.3)
     rec_m, simp_m, _, _ = run_scene(torch_cuda, b, oracle.GJK_EPA)
