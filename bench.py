@@ -430,7 +430,8 @@ def small_extras(rec: dict, batch, args) -> None:
     allocation, host staging, copies, kernel, copies back) and set it beside
     the reference's published kernel time."""
     import ogjk
-    ogjk.set_config(8, 32, PIPELINE)
+    lg, le = WORKLOADS["sphere32_gjk"]["lanes"]
+    ogjk.set_config(lg, le, PIPELINE)
     ogjk.collide_host(1, batch)  # warm-up
     reps = 20
     t = time.perf_counter()
@@ -484,6 +485,12 @@ def serving(args, headline: bool = True) -> dict:
     cnt = b.cnt.astype(np.int64)
     kpn = kp.cpu().numpy()
     k_bytes = float(12 * (cnt[kpn[:, 0]] + cnt[kpn[:, 1]]).sum() + 10_000 * (8 + 56 + 32 + 124))
+
+    if args.kernel_only:
+        # profiling: the 10,000-pair posed launch alone, no request loop
+        return {"value": 10_000 / (k_ms * 1e-3), "unit": "pairs/s", "steps": 20, "warmup": 2,
+                "ms_per_step": k_ms, "config": {"workload": "posed launch of 10000 pairs of the "
+                                                            "serving scene"}}
 
     def run_at(qps, rng_seed):
         srv = ogjk.Server(scene, max_batch_pairs=65536, slots=args.slots)
@@ -829,6 +836,8 @@ def main() -> int:
                     help="serving: launches in flight (at most the process's hardware queues, "
                          "4 by default: more slots share queues and block behind long launches)")
     ap.add_argument("--pairs", type=int, default=0, help="override the pair count")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="serving: time the 10,000-pair posed launch only (for profiling)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-FP64 comparison")
     ap.add_argument("--calibrate", action="store_true",
