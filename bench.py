@@ -492,17 +492,17 @@ def serving(args, headline: bool = True) -> dict:
                 "ms_per_step": k_ms, "config": {"workload": "posed launch of 10000 pairs of the "
                                                             "serving scene"}}
 
-    def run_at(qps, rng_seed):
+    def run_at(qps, rng_seed, n_run=n_req):
         srv = ogjk.Server(scene, max_batch_pairs=65536, slots=args.slots)
         rng = np.random.default_rng(rng_seed)
-        gaps = rng.exponential(1.0 / qps, size=n_req)
+        gaps = rng.exponential(1.0 / qps, size=n_run)
         arrive = np.cumsum(gaps)
-        tickets = [None] * n_req
+        tickets = [None] * n_run
         done_ev = threading.Event()
 
         def submitter():
             t0 = time.perf_counter()
-            for k in range(n_req):
+            for k in range(n_run):
                 while time.perf_counter() - t0 < arrive[k]:
                     pass
                 tickets[k] = srv.submit(*reqs[k])
@@ -511,8 +511,8 @@ def serving(args, headline: bool = True) -> dict:
         th = threading.Thread(target=submitter)
         t_start = time.perf_counter()
         th.start()
-        lats = np.zeros(n_req)
-        for k in range(n_req):
+        lats = np.zeros(n_run)
+        for k in range(n_run):
             while tickets[k] is None:
                 time.sleep(0)
             _, lat, _ = srv.wait(tickets[k])
@@ -523,10 +523,14 @@ def serving(args, headline: bool = True) -> dict:
         srv.close()
         return {"qps": qps, "p50_ms": float(np.percentile(lats, 50) * 1e3),
                 "p99_ms": float(np.percentile(lats, 99) * 1e3),
-                "max_ms": float(lats.max() * 1e3), "pairs_per_s": total_pairs / wall,
+                "max_ms": float(lats.max() * 1e3),
+                "pairs_per_s": sum(len(reqs[k][0]) for k in range(n_run)) / wall,
                 "launches": info["launches"], "slots": info["slots"], "wall_s": wall}
 
     rates = [250, 500, 2000, 4000, 6000] if headline else [2000, 4000, 6000]
+    # untimed pass first: first launches of every kernel shape the server
+    # uses, first use of a server's pinned staging
+    run_at(max(rates), 5, min(n_req, 400))
     sweep = []
     for qps in sorted(set([args.qps] + rates)):
         r = run_at(qps, 17 + qps)

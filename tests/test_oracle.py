@@ -140,6 +140,40 @@ def test_oracle64_invariants():
     assert np.allclose(rr.distances, 0.05, atol=2e-3), rr.distances[:10]
 
 
+def test_oracle64_epa_witnesses_pinned_by_geometry():
+    """EPA witnesses in FP64 against what the geometry fixes.  At convergence
+    (EPA.c:635, tolerance 100 * DBL_EPSILON) the closest face lies on the
+    boundary of A - B with normal n, so: w0 lies in conv(A) on A's supporting
+    plane along n, w1 in conv(B) on B's along -n, (w0 - w1).n = depth, and
+    where the origin projects inside that face (most pairs) w0 - w1 = depth n.
+    A swapped body index, a wrong barycentric order or sign moves the
+    witnesses off these planes or off depth n."""
+    b = gen.make(gen.SPHERE, 600, seed=37, nverts=48, box=1.6)
+    r = oracle.run(oracle.GJK_EPA, b.coords, b.off, b.cnt, b.pa, b.pb, bits=64, stats=True)
+    pen = np.where((r.distances < -1e-6) & (r.stats["epa_iters"] < 64))[0]
+    assert len(pen) > 150
+    w = r.simplices["witnesses"].astype(np.float64)
+    dirs = np.random.default_rng(5).normal(size=(32, 3))
+    tol = 1e-12
+    aligned = 0
+    for i in pen:
+        A = b.body(int(b.pa[i])).astype(np.float64)
+        B = b.body(int(b.pb[i])).astype(np.float64)
+        n = r.normals[i].astype(np.float64)
+        depth = -float(r.distances[i])
+        assert abs(np.linalg.norm(n) - 1.0) < 1e-14
+        hA, lB = (A @ n).max(), (B @ n).min()
+        assert -tol <= hA - w[i, 0] @ n <= tol, i
+        assert -tol <= w[i, 1] @ n - lB <= tol, i
+        assert abs((w[i, 0] - w[i, 1]) @ n - depth) <= tol, i
+        assert abs(depth - (hA - lB)) <= tol, i
+        # inside the convex hulls: no direction sees a witness beyond the body
+        assert np.all(dirs @ w[i, 0] <= (A @ dirs.T).max(axis=0) + tol), i
+        assert np.all(dirs @ w[i, 1] <= (B @ dirs.T).max(axis=0) + tol), i
+        aligned += np.abs(w[i, 0] - w[i, 1] - depth * n).max() <= 1e-9
+    assert aligned >= 0.8 * len(pen), (aligned, len(pen))
+
+
 def test_oracle64_matches_oracle32_within_fp32_error():
     b = gen.make(gen.SPHERE, 2000, seed=41, nverts=64, box=1.29)
     r64 = oracle.run(oracle.GJK, b.coords, b.off, b.cnt, b.pa, b.pb, bits=64)
