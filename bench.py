@@ -525,15 +525,18 @@ def serving(args, headline: bool = True) -> dict:
                 "p99_ms": float(np.percentile(lats, 99) * 1e3),
                 "max_ms": float(lats.max() * 1e3),
                 "pairs_per_s": sum(len(reqs[k][0]) for k in range(n_run)) / wall,
-                "launches": info["launches"], "slots": info["slots"], "wall_s": wall}
+                "launches": info["launches"], "slots": info["slots"], "wall_s": wall,
+                "requests": n_run}
 
     rates = [250, 500, 2000, 4000, 6000] if headline else [2000, 4000, 6000]
     # untimed pass first: first launches of every kernel shape the server
     # uses, first use of a server's pinned staging
     run_at(max(rates), 5, min(n_req, 400))
     sweep = []
+    # the stated rate runs args.requests (>= 10^4 by default); the others fewer
+    n_sweep = min(n_req, args.sweep_requests)
     for qps in sorted(set([args.qps] + rates)):
-        r = run_at(qps, 17 + qps)
+        r = run_at(qps, 17 + qps, n_req if qps == args.qps else n_sweep)
         sweep.append(r)
         log(f"[C4] qps {qps}: p50 {r['p50_ms']:.3f} ms p99 {r['p99_ms']:.3f} ms, "
             f"{r['pairs_per_s'] / 1e6:.2f} M pairs/s, {r['launches']} launches")
@@ -835,7 +838,10 @@ def main() -> int:
                     choices=["all"] + sorted(WORKLOADS) + ["serving", "broadphase", "scene_step",
                                                            "hulls"])
     ap.add_argument("--qps", type=int, default=1000, help="serving: request rate")
-    ap.add_argument("--requests", type=int, default=2000, help="serving: requests per rate")
+    ap.add_argument("--requests", type=int, default=10000,
+                    help="serving: requests at the stated rate (--qps)")
+    ap.add_argument("--sweep-requests", type=int, default=2000,
+                    help="serving: requests at each other rate of the sweep")
     ap.add_argument("--slots", type=int, default=4,
                     help="serving: launches in flight (at most the process's hardware queues, "
                          "4 by default: more slots share queues and block behind long launches)")

@@ -341,6 +341,8 @@ OGJK_DEV int best_bound(const ClusterBound* b, int n, float dx, float dy, float 
   return shfl_i<LPP>(bi, (int)__builtin_ctzll(group_ballot<LPP>(bub == gub) | (1ull << 63)));
 }
 
+// The clustered query through superclusters (TWO) or, without TWO, as one
+// level (the kernels use support_clustered1 below for that case).
 template <int LPP, bool TWO>
 OGJK_DEV Sup support_clustered(const CBody& A, const Query& q, int lane) {
   if (A.slot < 0) return support<LPP>(static_cast<const Body<float, 1>&>(A), q, lane);
@@ -421,9 +423,163 @@ OGJK_DEV Sup support_clustered(const CBody& A, const Query& q, int lane) {
   return finish1<LPP, false>(c, q);
 }
 
+// Calls f(k) for every slot of the clusters base + b for the set bits b of
+// m (a group ballot), each slot on exactly one lane.
+template <int LPP, class F>
+OGJK_DEV void visit_mask(const CBody& A, int base, uint64_t m, int lane, F&& f) {
+  while (m != 0) {
+    if constexpr (LPP >= kCluster) {
+      constexpr int PER = LPP / kCluster;
+      int cid = -1;
+#pragma unroll
+      for (int j = 0; j < PER; j++) {
+        if (m != 0) {
+          if (j == lane / kCluster) cid = (int)__builtin_ctzll(m);
+          m &= m - 1;
+        }
+      }
+      if (cid >= 0) {
+        const int k = (base + cid) * kCluster + (lane & (kCluster - 1));
+        if (k < A.n) f(k);
+      }
+    } else {
+      const int cid = (int)__builtin_ctzll(m);
+      m &= m - 1;
+#pragma unroll
+      for (int t = lane; t < kCluster; t += LPP) {
+        const int k = (base + cid) * kCluster + t;
+        if (k < A.n) f(k);
+      }
+    }
+  }
+}
+
+// One-level clustered query, for scenes without superclusters (the cluster
+// build gives superclusters to every body of more than kSuperMinClusters
+// clusters, so here a body has at most that many: K per lane).  Same
+// selection as support_clustered<LPP, false>, with fewer dependent memory
+// round trips: each lane keeps the bounds of its clusters in registers from
+// the first pass (the survivor test reads no bounds again), the best
+// cluster's sweep is the first step of the window sweep (it is not swept
+// twice), and each lane's input index is fetched before the final
+// reductions instead of after them.
+template <int LPP>
+OGJK_DEV Sup support_clustered1(const CBody& A, const Query& q, int lane) {
+  if (A.slot < 0) return support<LPP>(static_cast<const Body<float, 1>&>(A), q, lane);
+  constexpr int K = (kSuperMinClusters + LPP - 1) / LPP;
+  Sweep32 sw = sweep32_begin(q.dir);
+  const float dx = sw.dx, dy = sw.dy, dz = sw.dz;
+  const float dn = sqrtf(fmaf(dx, dx, fmaf(dy, dy, dz * dz))) * (1.0f + 0x1.0p-20f);
+  const int nc = (A.n + kCluster - 1) / kCluster;
+  const ClusterBound* cb = A.cv.bounds + A.slot / kCluster;
+  const float* sx = A.cv.x + A.slot;
+  const float* sy = A.cv.y + A.slot;
+  const float* sz = A.cv.z + A.slot;
+
+  // 1: every cluster's bound; the largest (lowest cluster on ties) is swept
+  // first.  Clusters past K per lane are not produced by the build but are
+  // handled (their bounds are recomputed in step 3).
+  float ub[K];
+  float bub = -INFINITY;
+  int bi = 0;
+#pragma unroll
+  for (int j = 0; j < K; j++) {
+    const int c = lane + j * LPP;
+    ub[j] = c < nc ? cluster_ub(cb[c], dx, dy, dz, dn) : -INFINITY;
+    if (ub[j] > bub) {
+      bub = ub[j];
+      bi = c;
+    }
+  }
+  for (int c = lane + K * LPP; c < nc; c += LPP) {
+    const float u = cluster_ub(cb[c], dx, dy, dz, dn);
+    if (u > bub) {
+      bub = u;
+      bi = c;
+    }
+  }
+  const float gub = group_max_f<LPP>(bub);
+  const int bc = shfl_i<LPP>(bi, (int)__builtin_ctzll(group_ballot<LPP>(bub == gub) | (1ull << 63)));
+
+  auto sweep = [&](int k) {
+    const float px = sx[k], py = sy[k], pz = sz[k];
+    const float d = fmaf(px, dx, fmaf(py, dy, pz * dz));
+    sw.second = fmaxf(sw.second, fminf(sw.best, d));
+    if (d > sw.best) {
+      sw.best = d;
+      sw.bi = k;
+      sw.bx = px;
+      sw.by = py;
+      sw.bz = pz;
+    }
+  };
+  // 2: the best cluster: its FP32 maximum m0 gives the threshold L
+#pragma unroll
+  for (int t = lane; t < kCluster; t += LPP) {
+    const int k = bc * kCluster + t;
+    if (k < A.n) sweep(k);
+  }
+  const float L = group_max_f<LPP>(sw.best) - 0x1.0p-15f * A.r1;
+
+  // 3: the other clusters whose bound reaches L (the visit order does not
+  // change the selection: ties are broken by input index below)
+  auto others = [&](auto&& f) {
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      const int c = lane + j * LPP;
+      visit_mask<LPP>(A, j * LPP, group_ballot<LPP>(c < nc && c != bc && !(ub[j] < L)), lane, f);
+    }
+    for (int base = K * LPP; base < nc; base += LPP) {
+      const int c = base + lane;
+      const bool live = c < nc && c != bc && !(cluster_ub(cb[c], dx, dy, dz, dn) < L);
+      visit_mask<LPP>(A, base, group_ballot<LPP>(live), lane, f);
+    }
+  };
+  others(sweep);
+
+  const int32_t* orig = A.cv.orig + A.slot;
+  const int oi = sw.bi >= 0 ? orig[sw.bi] : 0x7fffffff;
+  const float M = group_max_f<LPP>(sw.best);
+  const float T = M - 0x1.0p-19f * A.r1;
+  const bool multi = group_any<LPP>(sw.second >= T);
+
+  // FP64 re-rank of the window; ties go to the lowest input index
+  Cand c{-INFINITY, 0x7fffffff, D3{0.0, 0.0, 0.0}};
+  if (!multi) {
+    if (sw.bi >= 0 && sw.best >= T) {
+      c.p = D3{(double)sw.bx, (double)sw.by, (double)sw.bz};
+      c.v = dot(c.p, q.dir);
+      c.i = oi;
+    }
+  } else {
+    auto rank = [&](int k) {
+      const float px = sx[k], py = sy[k], pz = sz[k];
+      const float d = fmaf(px, dx, fmaf(py, dy, pz * dz));
+      if (d >= T) {
+        const D3 p{(double)px, (double)py, (double)pz};
+        const double v = dot(p, q.dir);
+        const int o = orig[k];
+        if (v > c.v || (v == c.v && o < c.i)) {
+          c.v = v;
+          c.i = o;
+          c.p = p;
+        }
+      }
+    };
+#pragma unroll
+    for (int t = lane; t < kCluster; t += LPP) {
+      const int k = bc * kCluster + t;
+      if (k < A.n) rank(k);
+    }
+    others(rank);
+  }
+  if (!(c.v == c.v)) c.v = -INFINITY;
+  return finish1<LPP, false>(c, q);
+}
+
 template <int LPP>
 OGJK_DEV Sup support(const CBody& A, const Query& q, int lane) {
-  return support_clustered<LPP, false>(A, q, lane);
+  return support_clustered1<LPP>(A, q, lane);
 }
 template <int LPP>
 OGJK_DEV Sup support(const CBody2& A, const Query& q, int lane) {

@@ -93,19 +93,29 @@ def posed_requests(n_bodies: int, n_requests: int, seed: int = 4, kmin: int = 1,
     """Serving requests over a scene of n_bodies: each has K ~ U{kmin..kmax}
     pairs (i, j), i != j uniform, and a fresh pose per pair side: unit
     quaternion (w, x, y, z) = normalised 4 x N(0,1), translation U[-tbox,
-    tbox]^3.  Returns a list of (pairs (K, 2) int32, poses (K, 14) float32)."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for _ in range(n_requests):
-        k = int(rng.integers(kmin, kmax + 1))
-        i = rng.integers(0, n_bodies, size=k)
-        j = (i + rng.integers(1, n_bodies, size=k)) % n_bodies
-        q = rng.normal(size=(k, 2, 4))
+    tbox]^3.  Returns a list of (pairs (K, 2) int32, poses (K, 14) float32).
+    Drawn 256 requests at a time, each chunk from its own stream spawned from
+    `seed` (the result does not depend on how many threads draw them)."""
+    from concurrent.futures import ThreadPoolExecutor
+    starts = list(range(0, n_requests, 256))
+    streams = np.random.SeedSequence(seed).spawn(len(starts))
+
+    def chunk(c):
+        rng = np.random.default_rng(streams[c])
+        ks = rng.integers(kmin, kmax + 1, size=min(256, n_requests - starts[c]))
+        m = int(ks.sum())
+        i = rng.integers(0, n_bodies, size=m)
+        j = (i + rng.integers(1, n_bodies, size=m)) % n_bodies
+        q = rng.normal(size=(m, 2, 4))
         q /= np.linalg.norm(q, axis=2, keepdims=True)
-        t = rng.uniform(-tbox, tbox, size=(k, 2, 3))
-        poses = np.concatenate([q, t], axis=2).reshape(k, 14).astype(np.float32)
-        out.append((np.stack([i, j], 1).astype(np.int32), poses))
-    return out
+        t = rng.uniform(-tbox, tbox, size=(m, 2, 3))
+        poses = np.concatenate([q, t], axis=2).reshape(m, 14).astype(np.float32)
+        pairs = np.stack([i, j], 1).astype(np.int32)
+        cut = np.cumsum(ks)[:-1]
+        return list(zip(np.split(pairs, cut), np.split(poses, cut)))
+
+    with ThreadPoolExecutor(min(16, max(1, len(os.sched_getaffinity(0))))) as ex:
+        return [r for part in ex.map(chunk, range(len(starts))) for r in part]
 
 
 def spheres(n: int, seed: int = 5, box: float = 10.0, rmin: float = 0.25, rmax: float = 1.0,
