@@ -342,7 +342,7 @@ OGJK_DEV int best_bound(const ClusterBound* b, int n, float dx, float dy, float 
 }
 
 // The clustered query through superclusters (TWO) or, without TWO, as one
-// level (the kernels use support_clustered1 below for that case).
+// level (used by groups of fewer than 64 lanes; see support(CBody) below).
 template <int LPP, bool TWO>
 OGJK_DEV Sup support_clustered(const CBody& A, const Query& q, int lane) {
   if (A.slot < 0) return support<LPP>(static_cast<const Body<float, 1>&>(A), q, lane);
@@ -477,8 +477,7 @@ OGJK_DEV Sup support_clustered1(const CBody& A, const Query& q, int lane) {
   const float* sz = A.cv.z + A.slot;
 
   // 1: every cluster's bound; the largest (lowest cluster on ties) is swept
-  // first.  Clusters past K per lane are not produced by the build but are
-  // handled (their bounds are recomputed in step 3).
+  // first.  Bounds past K per lane are recomputed in step 3.
   float ub[K];
   float bub = -INFINITY;
   int bi = 0;
@@ -577,9 +576,16 @@ OGJK_DEV Sup support_clustered1(const CBody& A, const Query& q, int lane) {
   return finish1<LPP, false>(c, q);
 }
 
+// The one-level query with cached bounds for whole-wave groups; narrower
+// groups would keep 128 / LPP bounds per lane, which costs the small-hull
+// kernels of the same scene more in spills than it saves.
 template <int LPP>
 OGJK_DEV Sup support(const CBody& A, const Query& q, int lane) {
-  return support_clustered1<LPP>(A, q, lane);
+  if constexpr (LPP >= 64) {
+    return support_clustered1<LPP>(A, q, lane);
+  } else {
+    return support_clustered<LPP, false>(A, q, lane);
+  }
 }
 template <int LPP>
 OGJK_DEV Sup support(const CBody2& A, const Query& q, int lane) {

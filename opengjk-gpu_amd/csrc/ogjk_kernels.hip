@@ -18,10 +18,11 @@
 #ifndef OGJK_GJK_WAVES_POSED
 #define OGJK_GJK_WAVES_POSED 3
 #endif
-// EPA: float scenes swept whole or with one level of clusters run faster at 3
-// waves/SIMD even though the compiler then spills 80-120 B/lane (100K x 64:
-// 1.22 -> 1.12 ms); superclustered, posed and the other sources spill more
-// and stay at 2 (10K x 100K: 6.56 vs 7.10 ms at 3).
+// EPA: float scenes swept whole, and the 64-lane kernel of scenes with one
+// level of clusters, run faster at 3 waves/SIMD even though the compiler then
+// spills 80-140 B/lane (100K x 64: 1.22 -> 1.12 ms); the narrower clustered
+// kernels (200+ B/lane at 3), superclustered, posed and the other sources
+// stay at 2 (10K x 100K: 6.56 vs 7.10 ms at 3).
 #ifndef OGJK_EPA_WAVES
 #define OGJK_EPA_WAVES 2
 #endif
@@ -55,8 +56,10 @@ struct SceneSrc {
   using BodyT =
       std::conditional_t<CL == 2, CBody2, std::conditional_t<CL == 1, CBody, Body<T, 1>>>;
   static constexpr int kGjkWaves = CL ? OGJK_GJK_WAVES_WIDE : OGJK_GJK_WAVES;
-  static constexpr int kEpaWaves =
-      std::is_same_v<T, float> && CL < 2 ? OGJK_EPA_WAVES_FLOAT : OGJK_EPA_WAVES;
+  template <int LPP>
+  static constexpr int kEpaWaves = std::is_same_v<T, float> && (CL == 0 || (CL == 1 && LPP == 64))
+                                       ? OGJK_EPA_WAVES_FLOAT
+                                       : OGJK_EPA_WAVES;
   // GJK stages small pairs in LDS (float scenes without clusters)
   static constexpr bool kStage = std::is_same_v<T, float> && CL == 0;
 
@@ -109,6 +112,7 @@ struct PoseSrc {
   int32_t by_body;  // poses[7 b] per body b, else poses[14 i], [14 i + 7] per pair side
   using BodyT = PBody;
   static constexpr int kGjkWaves = OGJK_GJK_WAVES_POSED;
+  template <int LPP>
   static constexpr int kEpaWaves = OGJK_EPA_WAVES;
   static constexpr bool kStage = false;
 
@@ -138,6 +142,7 @@ struct PolySrc {
   int32_t npool;
   using BodyT = Body<float, 3>;
   static constexpr int kGjkWaves = OGJK_GJK_WAVES;
+  template <int LPP>
   static constexpr int kEpaWaves = OGJK_EPA_WAVES;
   static constexpr bool kStage = false;
 
@@ -190,6 +195,7 @@ struct Aos64Src {
   const int32_t* pairs;
   using BodyT = Body<double, 3>;
   static constexpr int kGjkWaves = OGJK_GJK_WAVES;
+  template <int LPP>
   static constexpr int kEpaWaves = OGJK_EPA_WAVES;
   static constexpr bool kStage = false;
 
@@ -647,7 +653,7 @@ __global__ void __launch_bounds__(256, Src::kGjkWaves)
 // EPA over one class's job list, persistent: the groups of a wave claim jobs
 // together from sliced cursors until the list is exhausted.
 template <int LPP, class Src>
-__global__ void __launch_bounds__(256, Src::kEpaWaves)
+__global__ void __launch_bounds__(256, Src::template kEpaWaves<LPP>)
     k_epa_list(Src src, OutArgs out, BucketPlan plan, int cls, unsigned* c, const EpaJob* jobs) {
   const int GPB = (int)blockDim.x / LPP;  // groups per block: 1, 2 or 4 waves of them
   EpaShared* sh = (EpaShared*)ogjk_lds;
