@@ -18,11 +18,11 @@
 #ifndef OGJK_GJK_WAVES_POSED
 #define OGJK_GJK_WAVES_POSED 3
 #endif
-// EPA: float scenes swept whole, and the 64-lane kernel of scenes with one
-// level of clusters, run faster at 3 waves/SIMD even though the compiler then
-// spills 80-140 B/lane (100K x 64: 1.22 -> 1.12 ms); the narrower clustered
-// kernels (200+ B/lane at 3), superclustered, posed and the other sources
-// stay at 2 (10K x 100K: 6.56 vs 7.10 ms at 3).
+// EPA: float scenes swept whole run faster at 3 waves/SIMD even though the
+// compiler then spills 80 B/lane (100K x 64: 1.22 -> 1.12 ms); the 64-lane
+// kernels of clustered scenes fit 3 waves without spilling (the group is the
+// wave, so the pair's descriptors are scalar); narrower clustered kernels
+// (200+ B/lane at 3), posed and the other sources stay at 2.
 #ifndef OGJK_EPA_WAVES
 #define OGJK_EPA_WAVES 2
 #endif
@@ -60,6 +60,9 @@ struct SceneSrc {
   static constexpr int kEpaWaves = std::is_same_v<T, float> && (CL == 0 || (CL == 1 && LPP == 64))
                                        ? OGJK_EPA_WAVES_FLOAT
                                        : OGJK_EPA_WAVES;
+  // the EPA kernel of a superclustered scene is faster with the pair's
+  // descriptors in vector registers (10K x 100K: 4.7 vs 5.3 ms)
+  static constexpr bool kEpaScalarPair = CL != 2;
   // GJK stages small pairs in LDS (float scenes without clusters)
   static constexpr bool kStage = std::is_same_v<T, float> && CL == 0;
 
@@ -114,6 +117,7 @@ struct PoseSrc {
   static constexpr int kGjkWaves = OGJK_GJK_WAVES_POSED;
   template <int LPP>
   static constexpr int kEpaWaves = OGJK_EPA_WAVES;
+  static constexpr bool kEpaScalarPair = false;  // posed pairs: measured no faster
   static constexpr bool kStage = false;
 
   OGJK_DEV int pair_size(int i) const {
@@ -144,6 +148,7 @@ struct PolySrc {
   static constexpr int kGjkWaves = OGJK_GJK_WAVES;
   template <int LPP>
   static constexpr int kEpaWaves = OGJK_EPA_WAVES;
+  static constexpr bool kEpaScalarPair = false;
   static constexpr bool kStage = false;
 
   OGJK_DEV int pair_size(int i) const {
@@ -197,6 +202,7 @@ struct Aos64Src {
   static constexpr int kGjkWaves = OGJK_GJK_WAVES;
   template <int LPP>
   static constexpr int kEpaWaves = OGJK_EPA_WAVES;
+  static constexpr bool kEpaScalarPair = false;
   static constexpr bool kStage = false;
 
   OGJK_DEV int pair_size(int i) const {
@@ -621,7 +627,10 @@ __global__ void __launch_bounds__(256, Src::kGjkWaves)
   const bool multi = plan.nb > 1;
   const unsigned count =
       __builtin_amdgcn_readfirstlane(multi ? c[kCntBucket + bucket] : (unsigned)n);
-  const unsigned j = blockIdx.x * GPB + threadIdx.x / LPP;
+  unsigned j = blockIdx.x * GPB + threadIdx.x / LPP;
+  // a 64-lane group is the wave: say so, and the pair's descriptors load
+  // into scalar registers instead of taking vector registers
+  if constexpr (LPP == 64) j = __builtin_amdgcn_readfirstlane(j);
   if (j >= count) return;
   const int pair = multi ? list[__builtin_amdgcn_readfirstlane(c[kBucketStart + bucket]) + j]
                          : (int)j;
@@ -672,7 +681,10 @@ __global__ void __launch_bounds__(256, Src::template kEpaWaves<LPP>)
     Simplex s = {};
     Wit w = {};
     double dist;
-    const int pair = job_load(mine + j, s, w, dist);
+    int pair = job_load(mine + j, s, w, dist);
+    // a 64-lane group is the wave: the pair's descriptors can live in scalar
+    // registers
+    if constexpr (LPP == 64 && Src::kEpaScalarPair) pair = __builtin_amdgcn_readfirstlane(pair);
     typename Src::BodyT A, B;
     src.template load<LPP>(pair, lane, A, B);  // validated by the GJK kernel
     D3 normal{0.0, 0.0, 0.0};
