@@ -6,6 +6,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "ogjk_core.hpp"
@@ -15,9 +16,52 @@ namespace ogjk {
 
 namespace {
 
-// Index of (x, y) on the Hilbert curve over a 2^16 x 2^16 grid.
-OGJK_DEV uint32_t hilbert16(uint32_t x, uint32_t y) {
-  constexpr uint32_t n = 1u << 16;
+// Bodies of at most this many vertices are built by one block each, sorted in
+// LDS (at most 256 threads x 16 keys), in size classes of at most 64, 256,
+// 1024, 2048 and 4096 vertices.
+constexpr int kLdsBodyMax = 4096;
+constexpr int kClasses = 5;
+
+OGJK_DEV int body_class(int n) {
+  return n <= 64 ? 0 : (n <= 256 ? 1 : (n <= 1024 ? 2 : (n <= 2048 ? 3 : 4)));
+}
+
+// Per-class lists of the bodies built by one block each (0 < n <= lds_max):
+// lists[k * n_bodies + i].  One atomic per class per block; the order inside
+// a list is arbitrary (a body's output depends only on its own slots).
+__global__ void __launch_bounds__(256) k_body_classes(const int32_t* __restrict__ cnt,
+                                                      int n_bodies, int lds_max,
+                                                      unsigned* __restrict__ counts,
+                                                      int* __restrict__ lists) {
+  __shared__ unsigned hist[kClasses], base[kClasses];
+  if (threadIdx.x < kClasses) hist[threadIdx.x] = 0u;
+  __syncthreads();
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  int k = -1;
+  unsigned r = 0;
+  if (b < n_bodies) {
+    const int n = cnt[b];
+    if (n > 0 && n <= lds_max) {
+      k = body_class(n);
+      r = atomicAdd(&hist[k], 1u);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < kClasses)
+    base[threadIdx.x] = hist[threadIdx.x] ? atomicAdd(&counts[threadIdx.x], hist[threadIdx.x]) : 0u;
+  __syncthreads();
+  if (k >= 0) lists[(size_t)k * n_bodies + base[k] + r] = b;
+}
+
+// Sort keys: Hilbert index on a 2^kGridBits x 2^kGridBits grid over the
+// octahedral map of directions (4096^2 cells: far finer than a cluster of 32
+// vertices of the largest bodies), kKeyBits bits for the radix sorts.
+constexpr int kGridBits = 12;
+constexpr int kKeyBits = 2 * kGridBits;
+
+// Index of (x, y) on the Hilbert curve over the grid.
+OGJK_DEV uint32_t hilbert_key(uint32_t x, uint32_t y) {
+  constexpr uint32_t n = 1u << kGridBits;
   uint32_t d = 0;
   for (uint32_t s = n / 2; s > 0; s >>= 1) {
     const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
@@ -35,8 +79,8 @@ OGJK_DEV uint32_t hilbert16(uint32_t x, uint32_t y) {
   return d;
 }
 
-// Octahedral map of the direction v (any length) to [0, 2^16)^2.
-OGJK_DEV void octa16(float vx, float vy, float vz, uint32_t& qx, uint32_t& qy) {
+// Octahedral map of the direction v (any length) to the grid's cells.
+OGJK_DEV void octa_cell(float vx, float vy, float vz, uint32_t& qx, uint32_t& qy) {
   const float l1 = fabsf(vx) + fabsf(vy) + fabsf(vz);
   float u = 0.0f, w = 0.0f;
   if (l1 > 0.0f) {
@@ -49,9 +93,10 @@ OGJK_DEV void octa16(float vx, float vy, float vz, uint32_t& qx, uint32_t& qy) {
       w = ww;
     }
   }
-  const float fu = (u + 1.0f) * 32767.5f, fw = (w + 1.0f) * 32767.5f;
-  qx = fu > 0.0f ? (uint32_t)fminf(fu, 65535.0f) : 0u;  // NaN -> 0
-  qy = fw > 0.0f ? (uint32_t)fminf(fw, 65535.0f) : 0u;
+  constexpr float half = 0.5f * (float)(1u << kGridBits) - 0.5f, top = (float)((1u << kGridBits) - 1);
+  const float fu = (u + 1.0f) * half, fw = (w + 1.0f) * half;
+  qx = fu > 0.0f ? (uint32_t)fminf(fu, top) : 0u;  // NaN -> 0
+  qy = fw > 0.0f ? (uint32_t)fminf(fw, top) : 0u;
 }
 
 OGJK_DEV float block_min(float v, float* red) {
@@ -64,7 +109,7 @@ OGJK_DEV float block_min(float v, float* red) {
 
 OGJK_DEV float block_max(float v, float* red) { return -block_min(-v, red); }
 
-// One block per body: sort key of every vertex = Hilbert index of its
+// One block per body (blist[b0 + block]): sort key of every vertex = Hilbert index of its
 // direction from the centre of the body's bounding box (octahedral map), so
 // that consecutive vertices lie in the same narrow cone of directions, the
 // set a support query in nearby directions reaches; value = body-local index.
@@ -72,12 +117,13 @@ __global__ void __launch_bounds__(256) k_order_keys(const float* __restrict__ x,
                                                     const float* __restrict__ y,
                                                     const float* __restrict__ z,
                                                     const int64_t* __restrict__ off,
-                                                    const int32_t* __restrict__ cnt, int b0,
+                                                    const int32_t* __restrict__ cnt,
+                                                    const int* __restrict__ blist, int b0,
                                                     const int* __restrict__ kbeg,
                                                     uint32_t* __restrict__ keys,
                                                     int* __restrict__ vals) {
   __shared__ float red[4];
-  const int b = b0 + blockIdx.x;
+  const int b = blist[b0 + blockIdx.x];
   const int n = cnt[b];
   if (n <= 0) return;
   const int64_t o = off[b];
@@ -100,21 +146,21 @@ __global__ void __launch_bounds__(256) k_order_keys(const float* __restrict__ x,
   const int base = kbeg[blockIdx.x];
   for (int i = threadIdx.x; i < n; i += 256) {
     uint32_t qx, qy;
-    octa16(x[o + i] - c[0], y[o + i] - c[1], z[o + i] - c[2], qx, qy);
-    keys[base + i] = hilbert16(qx, qy);
+    octa_cell(x[o + i] - c[0], y[o + i] - c[1], z[o + i] - c[2], qx, qy);
+    keys[base + i] = hilbert_key(qx, qy);
     vals[base + i] = i;
   }
 }
 
-// One block per body: copy the vertices to their slots in sorted order; the
+// One block per body (blist[b0 + block]): copy the vertices to their slots in sorted order; the
 // padding slots up to the next multiple of kCluster get orig = -1.
 __global__ void __launch_bounds__(256) k_cluster_scatter(
     const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
     const int64_t* __restrict__ off, const int32_t* __restrict__ cnt,
-    const int64_t* __restrict__ slot, int b0, const int* __restrict__ kbeg,
-    const int* __restrict__ sorted, float* __restrict__ cx, float* __restrict__ cy,
-    float* __restrict__ cz, int32_t* __restrict__ orig) {
-  const int b = b0 + blockIdx.x;
+    const int64_t* __restrict__ slot, const int* __restrict__ blist, int b0,
+    const int* __restrict__ kbeg, const int* __restrict__ sorted, float* __restrict__ cx,
+    float* __restrict__ cy, float* __restrict__ cz, int32_t* __restrict__ orig) {
+  const int b = blist[b0 + blockIdx.x];
   const int n = cnt[b];
   if (n <= 0) return;
   const int64_t o = off[b], s = slot[b];
@@ -136,21 +182,12 @@ __global__ void __launch_bounds__(256) k_cluster_scatter(
   }
 }
 
-// One kCluster-lane slice per cluster: centre c = midpoint of the bounding
-// box (rounded to float), radius r = max |x - c| and half extents
-// h_k = max |x_k - c_k|, computed in FP64 and rounded up to float.
-__global__ void __launch_bounds__(256) k_cluster_bounds(const float* __restrict__ cx,
-                                                        const float* __restrict__ cy,
-                                                        const float* __restrict__ cz,
-                                                        const int32_t* __restrict__ orig,
-                                                        int64_t c0, int64_t n_clusters,
-                                                        ClusterBound* __restrict__ bounds) {
-  const int64_t c = c0 + ((int64_t)blockIdx.x * 256 + threadIdx.x) / kCluster;
-  const int t = threadIdx.x & (kCluster - 1);
-  const bool in = c < n_clusters;
-  const int64_t s = c * kCluster + t;
-  const bool valid = in && orig[s] >= 0;
-  const float p[3] = {valid ? cx[s] : 0.0f, valid ? cy[s] : 0.0f, valid ? cz[s] : 0.0f};
+// Bounds of one cluster held by a kCluster-lane slice, lane t holding slot t
+// (valid: a vertex, not padding): centre c = midpoint of the bounding box
+// (rounded to float), radius r = max |x - c| and half extents
+// h_k = max |x_k - c_k|, computed in FP64 and rounded up to float.  Every
+// lane of the slice returns the bound.
+OGJK_DEV ClusterBound slice_bound(bool valid, const float (&p)[3]) {
   float lo[3], hi[3];
 #pragma unroll
   for (int k = 0; k < 3; k++) {
@@ -179,13 +216,135 @@ __global__ void __launch_bounds__(256) k_cluster_bounds(const float* __restrict_
 #pragma unroll
     for (int k = 0; k < 3; k++) e[k] = fmax(e[k], __shfl_xor(e[k], m, kCluster));
   }
-  if (in && t == 0) {
-    ClusterBound cb;
-    cb.c = make_float4(ctr[0], ctr[1], ctr[2], nextafterf((float)r, INFINITY));
-    cb.h = make_float4(nextafterf((float)e[0], INFINITY), nextafterf((float)e[1], INFINITY),
-                       nextafterf((float)e[2], INFINITY), 0.0f);
-    bounds[c] = cb;
+  ClusterBound cb;
+  cb.c = make_float4(ctr[0], ctr[1], ctr[2], nextafterf((float)r, INFINITY));
+  cb.h = make_float4(nextafterf((float)e[0], INFINITY), nextafterf((float)e[1], INFINITY),
+                     nextafterf((float)e[2], INFINITY), 0.0f);
+  return cb;
+}
+
+// Minimum over a block of BLOCK threads (a multiple of the wave); red holds
+// BLOCK / 64 floats.
+template <int BLOCK>
+OGJK_DEV float group_min_blk(float v, float* red) {
+  for (int m = 32; m >= 1; m >>= 1) v = fminf(v, __shfl_xor(v, m, 64));
+  if constexpr (BLOCK == 64) {
+    return v;
+  } else {
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float r = red[0];
+#pragma unroll
+    for (int w = 1; w < BLOCK / 64; w++) r = fminf(r, red[w]);
+    return r;
   }
+}
+
+// Bodies of at most BLOCK * IPT vertices: one block per body (blist) builds
+// its whole part of the cluster copy -- the same keys as k_order_keys, a
+// stable sort in LDS, the copy in sorted order and every cluster's bound --
+// which is exactly what the batched path (keys, device segmented sort,
+// scatter, k_cluster_bounds) produces, in one pass over the body.  Small
+// bodies take one-wave blocks (no block barriers in the reductions).
+template <int BLOCK, int IPT>
+__global__ void __launch_bounds__(BLOCK) k_cluster_body(
+    const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
+    const int64_t* __restrict__ off, const int32_t* __restrict__ cnt,
+    const int64_t* __restrict__ slot, const int* __restrict__ blist, float* __restrict__ cx,
+    float* __restrict__ cy, float* __restrict__ cz, int32_t* __restrict__ orig,
+    ClusterBound* __restrict__ bounds) {
+  using Exchange = hipcub::BlockExchange<uint32_t, BLOCK, IPT>;
+  using Sort = hipcub::BlockRadixSort<uint32_t, BLOCK, IPT, int>;
+  __shared__ union {
+    typename Exchange::TempStorage ex;
+    typename Sort::TempStorage sort;
+  } tmp;
+  __shared__ float red[BLOCK / 64];
+  const int b = blist[blockIdx.x];
+  const int n = cnt[b];  // 0 < n <= BLOCK * IPT: the host's classes
+  const int64_t o = off[b], s = slot[b];
+  const int t = threadIdx.x;
+
+  // bounding-box centre, then the keys (item j of thread t: vertex j*BLOCK + t)
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+#pragma unroll
+  for (int j = 0; j < IPT; j++) {
+    const int i = j * BLOCK + t;
+    if (i < n) {
+      const float p[3] = {x[o + i], y[o + i], z[o + i]};
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        lo[k] = fminf(lo[k], p[k]);
+        hi[k] = fmaxf(hi[k], p[k]);
+      }
+    }
+  }
+  float c[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    lo[k] = group_min_blk<BLOCK>(lo[k], red);
+    hi[k] = -group_min_blk<BLOCK>(-hi[k], red);
+    c[k] = 0.5f * lo[k] + 0.5f * hi[k];
+  }
+  uint32_t key[IPT];
+  int val[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; j++) {
+    const int i = j * BLOCK + t;
+    key[j] = 0xffffffffu;  // padding sorts after every vertex (stable: after equal keys)
+    if (i < n) {
+      uint32_t qx, qy;
+      octa_cell(x[o + i] - c[0], y[o + i] - c[1], z[o + i] - c[2], qx, qy);
+      key[j] = hilbert_key(qx, qy);
+    }
+  }
+  // blocked order (thread t: items t*IPT ...) is index order, which a stable
+  // sort keeps among equal keys, as the segmented sort does
+  if constexpr (IPT > 1) {
+    Exchange(tmp.ex).StripedToBlocked(key);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < IPT; j++) val[j] = t * IPT + j;
+  Sort(tmp.sort).SortBlockedToStriped(key, val, 0, kKeyBits);
+
+  // rank j*BLOCK + t: the copy (coalesced) and, per 32-lane slice, one cluster
+  const int padded = (n + kCluster - 1) / kCluster * kCluster;
+#pragma unroll
+  for (int j = 0; j < IPT; j++) {
+    const int r = j * BLOCK + t;
+    if (j * BLOCK >= padded) break;  // uniform over the block
+    const bool valid = r < n;
+    const int l = val[j];
+    const float p[3] = {valid ? x[o + l] : 0.0f, valid ? y[o + l] : 0.0f,
+                        valid ? z[o + l] : 0.0f};
+    if (r < padded) {
+      cx[s + r] = p[0];
+      cy[s + r] = p[1];
+      cz[s + r] = p[2];
+      orig[s + r] = valid ? l : -1;
+    }
+    const ClusterBound cb = slice_bound(valid, p);
+    if ((t & (kCluster - 1)) == 0 && r < padded) bounds[(s + r) / kCluster] = cb;
+  }
+}
+
+// One kCluster-lane slice per cluster (the batched path's bounds).
+__global__ void __launch_bounds__(256) k_cluster_bounds(const float* __restrict__ cx,
+                                                        const float* __restrict__ cy,
+                                                        const float* __restrict__ cz,
+                                                        const int32_t* __restrict__ orig,
+                                                        int64_t c0, int64_t n_clusters,
+                                                        ClusterBound* __restrict__ bounds) {
+  const int64_t c = c0 + ((int64_t)blockIdx.x * 256 + threadIdx.x) / kCluster;
+  const int t = threadIdx.x & (kCluster - 1);
+  const bool in = c < n_clusters;
+  const int64_t s = c * kCluster + t;
+  const bool valid = in && orig[s] >= 0;
+  const float p[3] = {valid ? cx[s] : 0.0f, valid ? cy[s] : 0.0f, valid ? cz[s] : 0.0f};
+  const ClusterBound cb = slice_bound(valid, p);
+  if (in && t == 0) bounds[c] = cb;
 }
 
 OGJK_DEV double block_max_d(double v, double* red) {
@@ -315,20 +474,31 @@ hipError_t build_clusters(const float* x, const float* y, const float* z, const 
   if ((int64_t)sfirst.size() > n_supers || (int64_t)sfirst.size() > 0x7fffffff)
     return hipErrorInvalidValue;
   *n_supers_used = (int64_t)sfirst.size();
-  e = hipMemcpyAsync(body_super, hsuper.data(), sizeof(int64_t) * n_bodies,
-                     hipMemcpyHostToDevice, st);
+  e = sfirst.empty() ? hipMemsetAsync(body_super, 0xff, sizeof(int64_t) * n_bodies, st)  // all -1
+                     : hipMemcpyAsync(body_super, hsuper.data(), sizeof(int64_t) * n_bodies,
+                                      hipMemcpyHostToDevice, st);
   if (e != hipSuccess) return e;
 
-  // batches of consecutive bodies
+  // Bodies of at most lds_max vertices are built whole by one block each
+  // (k_cluster_body, three size classes); larger ones go through batches of
+  // keys, a device segmented sort and a scatter.  OGJK_CLUSTER_LDS_MAX=0 sends
+  // every body through the batches (the two paths give identical arrays).
+  int lds_max = kLdsBodyMax;
+  if (const char* v = getenv("OGJK_CLUSTER_LDS_MAX"))
+    lds_max = std::max(0, std::min(atoi(v), kLdsBodyMax));
+  std::vector<int> large;  // the batched bodies
+  for (int b = 0; b < n_bodies; b++)
+    if (hcnt[b] > lds_max) large.push_back(b);
+  // batches of consecutive large bodies (positions in `large`)
   std::vector<int> bstart{0};
-  std::vector<int> kbeg_h;  // batch-local key offset of every body, batch after batch
+  std::vector<int> kbeg_h;  // batch-local key offset of every large body, batch after batch
   std::vector<int64_t> bverts;
   {
     int64_t acc = 0;
-    for (int b = 0; b < n_bodies; b++) {
-      const int64_t n = hcnt[b] > 0 ? hcnt[b] : 0;
+    for (int q = 0; q < (int)large.size(); q++) {
+      const int64_t n = hcnt[large[q]];
       if (acc + n > kBatch) {
-        bstart.push_back(b);
+        bstart.push_back(q);
         bverts.push_back(acc);
         acc = 0;
       }
@@ -336,7 +506,7 @@ hipError_t build_clusters(const float* x, const float* y, const float* z, const 
       acc += n;
     }
     bverts.push_back(acc);
-    bstart.push_back(n_bodies);
+    bstart.push_back((int)large.size());
   }
   int64_t maxv = 0;
   int maxb = 0;
@@ -347,7 +517,8 @@ hipError_t build_clusters(const float* x, const float* y, const float* z, const 
   maxv = std::max<int64_t>(maxv, 1);
 
   uint32_t *keys = nullptr, *keys2 = nullptr;
-  int *vals = nullptr, *vals2 = nullptr, *kbeg = nullptr;
+  int *vals = nullptr, *vals2 = nullptr, *kbeg = nullptr, *blist = nullptr, *lbig = nullptr;
+  unsigned* ccount = nullptr;
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
   auto cleanup = [&]() {
@@ -356,15 +527,57 @@ hipError_t build_clusters(const float* x, const float* y, const float* z, const 
     (void)hipFree(vals);
     (void)hipFree(vals2);
     (void)hipFree(kbeg);
+    (void)hipFree(blist);
+    (void)hipFree(lbig);
+    (void)hipFree(ccount);
     (void)hipFree(tmp);
   };
-  if ((e = hipMalloc(&keys, 4 * maxv)) != hipSuccess || (e = hipMalloc(&keys2, 4 * maxv)) ||
-      (e = hipMalloc(&vals, 4 * maxv)) || (e = hipMalloc(&vals2, 4 * maxv)) ||
-      (e = hipMalloc(&kbeg, sizeof(int) * ((size_t)maxb + 1)))) {
+  // the one-block bodies: class lists built on the device
+  unsigned hcount[kClasses] = {};
+  if (lds_max > 0) {
+    if ((e = hipMalloc(&blist, sizeof(int) * (size_t)kClasses * n_bodies)) != hipSuccess ||
+        (e = hipMalloc(&ccount, sizeof(unsigned) * kClasses)) != hipSuccess ||
+        (e = hipMemsetAsync(ccount, 0, sizeof(unsigned) * kClasses, st)) != hipSuccess) {
+      cleanup();
+      return e;
+    }
+    hipLaunchKernelGGL(k_body_classes, dim3((unsigned)((n_bodies + 255) / 256)), dim3(256), 0, st,
+                       cnt, n_bodies, lds_max, ccount, blist);
+    if ((e = hipMemcpyAsync(hcount, ccount, sizeof(hcount), hipMemcpyDeviceToHost, st)) !=
+            hipSuccess ||
+        (e = hipStreamSynchronize(st)) != hipSuccess) {
+      cleanup();
+      return e;
+    }
+  }
+  {
+    ClusterBound* cb = (ClusterBound*)bounds;
+    auto launch = [&](auto kernel, int k, int block) {
+      if (hcount[k] > 0)
+        hipLaunchKernelGGL(kernel, dim3(hcount[k]), dim3(block), 0, st, x, y, z, off, cnt,
+                           body_slot, blist + (size_t)k * n_bodies, cx, cy, cz, orig, cb);
+    };
+    launch(k_cluster_body<64, 1>, 0, 64);
+    launch(k_cluster_body<64, 4>, 1, 64);
+    launch(k_cluster_body<256, 4>, 2, 256);
+    launch(k_cluster_body<256, 8>, 3, 256);  // 512-thread blocks measured slower
+    launch(k_cluster_body<256, 16>, 4, 256);
+    if ((e = hipGetLastError()) != hipSuccess) {
+      cleanup();
+      return e;
+    }
+  }
+  if (!large.empty() &&
+      ((e = hipMalloc(&keys, 4 * maxv)) != hipSuccess || (e = hipMalloc(&keys2, 4 * maxv)) ||
+       (e = hipMalloc(&vals, 4 * maxv)) || (e = hipMalloc(&vals2, 4 * maxv)) ||
+       (e = hipMalloc(&kbeg, sizeof(int) * ((size_t)maxb + 1))) ||
+       (e = hipMalloc(&lbig, sizeof(int) * large.size())) ||
+       (e = hipMemcpyAsync(lbig, large.data(), sizeof(int) * large.size(), hipMemcpyHostToDevice,
+                           st)))) {
     cleanup();
     return e;
   }
-  for (size_t i = 0; i + 1 < bstart.size(); i++) {
+  for (size_t i = 0; i + 1 < bstart.size() && !large.empty(); i++) {
     const int b0 = bstart[i], nb = bstart[i + 1] - b0;
     const int nv = (int)bverts[i];
     if (nb == 0) continue;
@@ -376,11 +589,11 @@ hipError_t build_clusters(const float* x, const float* y, const float* z, const 
       cleanup();
       return e;
     }
-    hipLaunchKernelGGL(k_order_keys, dim3(nb), dim3(256), 0, st, x, y, z, off, cnt, b0, kbeg, keys,
-                       vals);
+    hipLaunchKernelGGL(k_order_keys, dim3(nb), dim3(256), 0, st, x, y, z, off, cnt, lbig, b0, kbeg,
+                       keys, vals);
     size_t need = 0;
     e = hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, need, keys, keys2, vals, vals2, nv,
-                                                    nb, kbeg, kbeg + 1, 0, 32, st);
+                                                    nb, kbeg, kbeg + 1, 0, kKeyBits, st);
     if (e != hipSuccess) {
       cleanup();
       return e;
@@ -395,13 +608,13 @@ hipError_t build_clusters(const float* x, const float* y, const float* z, const 
       tmp_bytes = need;
     }
     e = hipcub::DeviceSegmentedRadixSort::SortPairs(tmp, need, keys, keys2, vals, vals2, nv, nb,
-                                                    kbeg, kbeg + 1, 0, 32, st);
+                                                    kbeg, kbeg + 1, 0, kKeyBits, st);
     if (e != hipSuccess) {
       cleanup();
       return e;
     }
     hipLaunchKernelGGL(k_cluster_scatter, dim3(nb), dim3(256), 0, st, x, y, z, off, cnt, body_slot,
-                       b0, kbeg, vals2, cx, cy, cz, orig);
+                       lbig, b0, kbeg, vals2, cx, cy, cz, orig);
     // the host vectors of this batch die at the end of the iteration
     if ((e = hipStreamSynchronize(st)) != hipSuccess) {
       cleanup();
@@ -409,8 +622,10 @@ hipError_t build_clusters(const float* x, const float* y, const float* z, const 
     }
   }
   const int64_t n_clusters = total / kCluster;
-  // 2^24 clusters (2^29 work-items) per dispatch: a dispatch holds < 2^32
-  for (int64_t c0 = 0; c0 < n_clusters; c0 += (int64_t)1 << 24) {
+  // bounds of the batched bodies' clusters (recomputed for every cluster: the
+  // blocks of k_cluster_body give the same values); 2^24 clusters (2^29
+  // work-items) per dispatch: a dispatch holds < 2^32
+  for (int64_t c0 = 0; c0 < n_clusters && !large.empty(); c0 += (int64_t)1 << 24) {
     const int64_t nc = n_clusters - c0 < ((int64_t)1 << 24) ? n_clusters - c0 : ((int64_t)1 << 24);
     hipLaunchKernelGGL(k_cluster_bounds, dim3((unsigned)((nc * kCluster + 255) / 256)), dim3(256),
                        0, st, cx, cy, cz, orig, c0, n_clusters, (ClusterBound*)bounds);

@@ -127,6 +127,59 @@ def test_cluster_layout(torch_cuda):
                 assert (np.abs(q - ctr) <= h).all()
 
 
+def _cluster_arrays(torch, batch, env):
+    """Builds the scene's clusters with OGJK_CLUSTER_LDS_MAX=env (None: unset)
+    and returns every array the build writes, cut to the filled part."""
+    import os
+    old = os.environ.pop("OGJK_CLUSTER_LDS_MAX", None)
+    try:
+        if env is not None:
+            os.environ["OGJK_CLUSTER_LDS_MAX"] = env
+        sc = ogjk.DeviceScene(batch)
+        sc.build_clusters(1)
+        torch.cuda.synchronize()
+    finally:
+        os.environ.pop("OGJK_CLUSTER_LDS_MAX", None)
+        if old is not None:
+            os.environ["OGJK_CLUSTER_LDS_MAX"] = old
+    cnt = batch.cnt.astype(np.int64)
+    total = int((-(-cnt // 32) * 32).sum())
+    ns = int(sc.clusters.n_supers_used)
+    bits = lambda t, n: t.cpu().numpy()[:n].view(np.uint32)  # noqa: E731
+    return {"x": bits(sc.cx, total), "y": bits(sc.cy, total), "z": bits(sc.cz, total),
+            "orig": bits(sc.corig, total), "bounds": bits(sc.cbounds, total // 32 * 8),
+            "slot": sc.cslot.cpu().numpy(), "super": sc.csuper.cpu().numpy(),
+            "sbounds": bits(sc.csbounds, ns * 8), "n_supers": ns}
+
+
+def test_cluster_build_paths_identical(torch_cuda):
+    """Bodies of at most 4,096 vertices are built by one block each (keys, a
+    stable sort in LDS, copy and bounds in one kernel); the batched path (keys,
+    device segmented sort, scatter, bounds kernel) must give the same arrays
+    bit for bit.  Sizes straddle the block classes (1024, 2048, 4096) and the
+    batched bodies; repeated points give equal keys, so both sorts must keep
+    index order among them."""
+    rng = np.random.default_rng(12)
+    sizes = [1, 2, 3, 31, 32, 33, 255, 256, 257, 1023, 1024, 1025, 2047, 2048, 2049, 4095, 4096,
+             4097, 5000, 9000]
+    A, B = [], []
+    for n in sizes:
+        for side in (A, B):
+            p = rng.normal(size=(n, 3)) * rng.uniform(0.2, 2.0, size=3)
+            if n > 8:  # repeated points: equal sort keys
+                k = max(1, n // 10)
+                p[rng.integers(0, n, size=k)] = p[rng.integers(0, n, size=k)]
+            side.append(p.astype(np.float32))
+    mixed = gen.from_bodies(A, B)
+    small = gen.make(gen.ELLIPSOID, 300, seed=7, box=1.5, nmin=4, nmax=4096)
+    for name, batch in (("mixed sizes", mixed), ("ellipsoids <= 4096", small)):
+        ref = _cluster_arrays(torch_cuda, batch, "0")
+        new = _cluster_arrays(torch_cuda, batch, None)
+        assert ref["n_supers"] == new["n_supers"], name
+        for k in ref:
+            assert np.array_equal(ref[k], new[k]), f"{name}: {k} differs"
+
+
 # Size-bucketed launch tables (device partition, per-bucket lanes, EPA lane
 # classes), with and without clusters, against the oracle.
 @pytest.mark.parametrize("table", [([64], [8, 64], [32, 64]), ([1], [32, 64], [32, 64]),
