@@ -462,7 +462,10 @@ def serving(args, headline: bool = True) -> dict:
     import torch
     import ogjk
     from ogjk import gen
-    ogjk.set_config(32, 64, PIPELINE)  # hulls of up to 512 vertices: measured best
+    lg, le = (int(x) for x in args.lanes.split(",")) if args.lanes else (64, 64)
+    # 64 / 64 lanes for hulls of up to 512 vertices: kernel of 10,000 posed pairs 0.188 ms, 0.212
+    # at 32 / 64, 0.261 at 16 / 64, 0.253 at 32 / 32
+    ogjk.set_config(lg, le, PIPELINE)
     b = gen.make(gen.ELLIPSOID, 10_000, seed=4, box=0.0, nmin=8, nmax=512, nthreads=cores())
     scene = ogjk.DeviceScene(b)
     nb = len(b.cnt)
@@ -548,7 +551,7 @@ def serving(args, headline: bool = True) -> dict:
         "config": {"workload": "serving: 20000-body scene (8-512 vertices log-uniform), Poisson "
                                f"requests at {args.qps} QPS, K ~ U{{1..10000}} pairs, fresh pose per "
                                "pair side, t ~ U[-10,10]^3", "requests": n_req,
-                   "pairs": total_pairs, "lanes_per_pair": {"gjk": 32, "epa": 64},
+                   "pairs": total_pairs, "lanes_per_pair": {"gjk": lg, "epa": le},
                    "server": {"slots": main_r["slots"], "max_batch_pairs": 65536}},
         "latency_ms": {"p50": main_r["p50_ms"], "p99": main_r["p99_ms"], "max": main_r["max_ms"],
                        "qps": args.qps, "target_p99_ms": 2.0},
@@ -671,7 +674,8 @@ def scene_step(args) -> dict:
     import oracle
     import ogjk
     from ogjk import gen
-    ogjk.set_config(32, 64, PIPELINE)  # broad-phase pairs are close: measured best
+    lg, le = (int(x) for x in args.lanes.split(",")) if args.lanes else (64, 64)
+    ogjk.set_config(lg, le, PIPELINE)  # 64 / 64 for broad-phase pairs: measured best
     b = gen.make(gen.ELLIPSOID, 10_000, seed=4, box=0.0, nmin=8, nmax=512, nthreads=cores())
     nb = len(b.cnt)
     radii = gen.body_radii(b.coords, b.off, b.cnt)
@@ -701,7 +705,7 @@ def scene_step(args) -> dict:
         "dtype": "f32+f64",
         "config": {"workload": f"scene step: {nb} bodies (8-512 vertices log-uniform), one pose "
                                f"per body, t ~ U[-{tbox:.0f},{tbox:.0f}]^3, {grid}",
-                   "bodies": nb, "pairs": n, "lanes_per_pair": {"gjk": 32, "epa": 64}},
+                   "bodies": nb, "pairs": n, "lanes_per_pair": {"gjk": lg, "epa": le}},
         "narrow_phase_ms": float(np.mean(nms)),
         "broad_phase_ms": float(np.mean(ms) - np.mean(nms)),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -853,7 +857,8 @@ def main() -> int:
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 256 MiB elementwise copy (reads and writes 256 MiB) so a "
                          "profiler's FETCH_SIZE/WRITE_SIZE can be scaled against known bytes")
-    ap.add_argument("--lanes", default=None, help="GJK,EPA lanes per pair (headline workload)")
+    ap.add_argument("--lanes", default=None,
+                    help="GJK,EPA lanes per pair (a single batch workload, serving, scene_step)")
     ap.add_argument("--cluster-min", type=int, default=None,
                     help="cull bodies with at least this many vertices (0: no clusters; "
                          "default per workload)")
