@@ -152,33 +152,58 @@ __global__ void __launch_bounds__(256) k_order_keys(const float* __restrict__ x,
   }
 }
 
-// One block per body (blist[b0 + block]): copy the vertices to their slots in sorted order; the
-// padding slots up to the next multiple of kCluster get orig = -1.
+// The batch's sorted positions, one thread each: position p of the batch's
+// concatenated segments is vertex i = p - kbeg[q] of body blist[b0 + q]
+// (q by binary search); its slot gets the vertex sorted there.  Blocks are
+// remapped so each XCD (blocks i, i + 8, ...) takes one contiguous eighth of
+// the positions: an XCD then gathers from about one body at a time, whose
+// vertices stay in its L2 (one block per body gathered from many bodies at
+// once and missed in L2 on nearly every load).
 __global__ void __launch_bounds__(256) k_cluster_scatter(
     const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
-    const int64_t* __restrict__ off, const int32_t* __restrict__ cnt,
-    const int64_t* __restrict__ slot, const int* __restrict__ blist, int b0,
-    const int* __restrict__ kbeg, const int* __restrict__ sorted, float* __restrict__ cx,
-    float* __restrict__ cy, float* __restrict__ cz, int32_t* __restrict__ orig) {
-  const int b = blist[b0 + blockIdx.x];
+    const int64_t* __restrict__ off, const int64_t* __restrict__ slot,
+    const int* __restrict__ blist, int b0, int nb, const int* __restrict__ kbeg, int nv,
+    const int* __restrict__ sorted, float* __restrict__ cx, float* __restrict__ cy,
+    float* __restrict__ cz, int32_t* __restrict__ orig) {
+  // the grid is a multiple of 8 blocks, so this is a bijection of [0, grid)
+  const unsigned per = gridDim.x / 8u;
+  const unsigned lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  const int64_t p = (int64_t)lb * 256 + threadIdx.x;
+  if (p >= nv) return;
+  int lo = 0, hi = nb - 1;  // last q with kbeg[q] <= p
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (kbeg[mid] <= p) lo = mid;
+    else hi = mid - 1;
+  }
+  const int b = blist[b0 + lo];
+  const int i = (int)(p - kbeg[lo]);
+  const int l = sorted[p];
+  const int64_t o = off[b], s = slot[b] + i;
+  cx[s] = x[o + l];
+  cy[s] = y[o + l];
+  cz[s] = z[o + l];
+  orig[s] = l;
+}
+
+// The padding slots of a body, up to the next multiple of kCluster: zeros and
+// orig = -1 (one thread per body of the batch).
+__global__ void __launch_bounds__(256) k_cluster_pad(const int32_t* __restrict__ cnt,
+                                                     const int64_t* __restrict__ slot,
+                                                     const int* __restrict__ blist, int b0, int nb,
+                                                     float* __restrict__ cx, float* __restrict__ cy,
+                                                     float* __restrict__ cz,
+                                                     int32_t* __restrict__ orig) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= nb) return;
+  const int b = blist[b0 + q];
   const int n = cnt[b];
-  if (n <= 0) return;
-  const int64_t o = off[b], s = slot[b];
-  const int base = kbeg[blockIdx.x];
   const int padded = (n + kCluster - 1) / kCluster * kCluster;
-  for (int i = threadIdx.x; i < padded; i += 256) {
-    if (i < n) {
-      const int l = sorted[base + i];
-      cx[s + i] = x[o + l];
-      cy[s + i] = y[o + l];
-      cz[s + i] = z[o + l];
-      orig[s + i] = l;
-    } else {
-      cx[s + i] = 0.0f;
-      cy[s + i] = 0.0f;
-      cz[s + i] = 0.0f;
-      orig[s + i] = -1;
-    }
+  for (int i = n; i < padded; i++) {
+    cx[slot[b] + i] = 0.0f;
+    cy[slot[b] + i] = 0.0f;
+    cz[slot[b] + i] = 0.0f;
+    orig[slot[b] + i] = -1;
   }
 }
 
@@ -613,8 +638,11 @@ hipError_t build_clusters(const float* x, const float* y, const float* z, const 
       cleanup();
       return e;
     }
-    hipLaunchKernelGGL(k_cluster_scatter, dim3(nb), dim3(256), 0, st, x, y, z, off, cnt, body_slot,
-                       lbig, b0, kbeg, vals2, cx, cy, cz, orig);
+    const unsigned sblocks = ((unsigned)((nv + 255) / 256) + 7u) / 8u * 8u;  // a multiple of 8
+    hipLaunchKernelGGL(k_cluster_scatter, dim3(sblocks), dim3(256), 0, st, x, y, z, off, body_slot,
+                       lbig, b0, nb, kbeg, nv, vals2, cx, cy, cz, orig);
+    hipLaunchKernelGGL(k_cluster_pad, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, cnt,
+                       body_slot, lbig, b0, nb, cx, cy, cz, orig);
     // the host vectors of this batch die at the end of the iteration
     if ((e = hipStreamSynchronize(st)) != hipSuccess) {
       cleanup();
