@@ -720,11 +720,17 @@ OGJK_DEV Sup swap_halves(const Sup& r) {
 // slower on the uneven 1M-pair workload, 8% faster on 64-vertex pairs).
 template <class BT>
 constexpr bool kSplitQueries = !std::is_base_of_v<CBody, BT>;
+// Posed bodies in a 64-lane group (the wave) also query one after the other:
+// each query's pose is then wave-uniform and stays in scalar registers (the
+// posed GJK kernel's scratch 60 -> 12 B/lane; 10,000 posed pairs 0.186 ->
+// 0.181 ms).
+template <int LPP, class BT>
+constexpr bool kSplit = kSplitQueries<BT> && !(LPP == 64 && std::is_same_v<BT, PBody>);
 
 template <int LPP, class BT>
 OGJK_DEV void support_pair(const BT& A, const Query& qa, const BT& B, const Query& qb, int lane,
                            Sup& ra, Sup& rb) {
-  if constexpr (LPP >= 8 && kSplitQueries<BT>) {
+  if constexpr (LPP >= 8 && kSplit<LPP, BT>) {
     // the lower half of the group answers A's query, the upper half B's, at
     // the same time; then each half takes the other's result
     const bool hi = lane >= LPP / 2;
