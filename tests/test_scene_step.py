@@ -21,9 +21,13 @@ def torch_cuda():
     return torch
 
 
-@pytest.mark.parametrize("tbox,max_grid", [(12.0, 30), (20.0, 64)])
-def test_scene_step_parity(torch_cuda, tbox, max_grid):
+@pytest.mark.parametrize("tbox,max_grid,lanes",
+                         [(12.0, 30, (8, 32)), (20.0, 64, (8, 32)), (12.0, 30, (64, 64))])
+def test_scene_step_parity(torch_cuda, tbox, max_grid, lanes):
+    """lanes (64, 64): one pose per body in wave-wide groups, whose support
+    queries run one after the other."""
     torch = torch_cuda
+    ogjk.set_config(lanes[0], lanes[1], 0)
     b = gen.make(gen.ELLIPSOID, 1000, seed=51, box=0.0, nmin=8, nmax=512)
     nb = len(b.cnt)
     radii = gen.body_radii(b.coords, b.off, b.cnt)
@@ -31,8 +35,11 @@ def test_scene_step_parity(torch_cuda, tbox, max_grid):
     grid = ogjk.Grid.for_spheres(float(radii.max()), tbox, max_grid)
     sc = ogjk.DeviceScene(b)
     step = ogjk.SceneStep(sc, radii, grid, max_pairs=20 * nb, f64_record=True)
-    pairs, found, out = step(torch.from_numpy(poses).cuda())
-    torch.cuda.synchronize()
+    try:
+        pairs, found, out = step(torch.from_numpy(poses).cuda())
+        torch.cuda.synchronize()
+    finally:
+        ogjk.set_config(8, 32, 0)  # the library's default launch shape
     got = pairs.cpu().numpy()
     spheres = np.concatenate([poses[:, 4:7], radii[:, None]], 1)
     ref_pairs, rfound = oracle.broad_phase(spheres, grid.cell_size, grid.boundary, grid.grid_size)

@@ -48,6 +48,29 @@ def test_posed_server_parity(torch_cuda, scene):
     assert res["exact"] == res["n"], res["inexact"]
 
 
+@pytest.mark.parametrize("lanes", [(16, 32), (32, 64), (64, 64), (64, 32)])
+def test_posed_parity_lanes(torch_cuda, scene, lanes):
+    """Posed pairs at each launch shape against the oracle: 64-lane groups
+    run the two support queries one after the other (wave-uniform poses),
+    narrower ones split the group between them."""
+    torch = torch_cuda
+    b, sc = scene
+    pairs, poses = gen.posed_requests(len(b.cnt), 1, seed=46, kmin=2000, kmax=2000, tbox=1.5)[0]
+    ogjk.set_config(lanes[0], lanes[1], 0)
+    try:
+        dev = sc.collide_posed(oracle.GJK_EPA, torch.from_numpy(pairs).cuda(),
+                               torch.from_numpy(poses).cuda(), f64_record=True)
+        torch.cuda.synchronize()
+    finally:
+        ogjk.set_config(8, 32, 0)  # the library's default launch shape
+    ref = oracle_posed(b, pairs, poses)
+    res = compare_f64(dev["f64"].cpu().numpy(), ogjk.simplices_from_device(dev["simplices"]), ref,
+                      oracle.GJK_EPA)
+    print(f"\n[posed lanes {lanes}] {res}", flush=True)
+    assert (ref.distances < 0).mean() > 0.1  # EPA exercised
+    assert res["exact"] == res["n"], res["inexact"]
+
+
 def test_posed_device_equals_server(torch_cuda, scene):
     torch = torch_cuda
     b, sc = scene
