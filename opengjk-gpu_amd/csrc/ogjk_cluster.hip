@@ -228,19 +228,22 @@ OGJK_DEV ClusterBound slice_bound(bool valid, const float (&p)[3]) {
     }
   }
   float ctr[3];
-  double e[3], r = 0.0;
+  double e[3], r2 = 0.0;
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     ctr[k] = 0.5f * lo[k] + 0.5f * hi[k];
     e[k] = valid ? fabs((double)p[k] - (double)ctr[k]) : 0.0;
   }
-  if (valid) r = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+  if (valid) r2 = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
 #pragma unroll
   for (int m = 1; m < kCluster; m <<= 1) {
-    r = fmax(r, __shfl_xor(r, m, kCluster));
+    r2 = fmax(r2, __shfl_xor(r2, m, kCluster));
 #pragma unroll
     for (int k = 0; k < 3; k++) e[k] = fmax(e[k], __shfl_xor(e[k], m, kCluster));
   }
+  // a correctly rounded square root is monotone: the root of the largest
+  // squared distance is the largest root, bit for bit
+  const double r = sqrt(r2);
   ClusterBound cb;
   cb.c = make_float4(ctr[0], ctr[1], ctr[2], nextafterf((float)r, INFINITY));
   cb.h = make_float4(nextafterf((float)e[0], INFINITY), nextafterf((float)e[1], INFINITY),
@@ -411,18 +414,18 @@ __global__ void __launch_bounds__(256) k_super_bounds(const float* __restrict__ 
     hi[k] = block_max(hi[k], red);
     ctr[k] = 0.5f * lo[k] + 0.5f * hi[k];
   }
-  double r = 0.0, e[3] = {0.0, 0.0, 0.0};
+  double r2 = 0.0, e[3] = {0.0, 0.0, 0.0};
   for (int i = threadIdx.x; i < n; i += 256) {
     if (orig[s0 + i] < 0) continue;
     const double ex = fabs((double)cx[s0 + i] - (double)ctr[0]);
     const double ey = fabs((double)cy[s0 + i] - (double)ctr[1]);
     const double ez = fabs((double)cz[s0 + i] - (double)ctr[2]);
-    r = fmax(r, sqrt(ex * ex + ey * ey + ez * ez));
+    r2 = fmax(r2, ex * ex + ey * ey + ez * ez);
     e[0] = fmax(e[0], ex);
     e[1] = fmax(e[1], ey);
     e[2] = fmax(e[2], ez);
   }
-  r = block_max_d(r, redd);
+  const double r = sqrt(block_max_d(r2, redd));  // as in slice_bound
 #pragma unroll
   for (int k = 0; k < 3; k++) e[k] = block_max_d(e[k], redd);
   if (threadIdx.x == 0) {
