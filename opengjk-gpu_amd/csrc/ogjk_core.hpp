@@ -499,6 +499,37 @@ OGJK_DEV Sup support_clustered1(const CBody& A, const Query& q, int lane) {
   }
   const float gub = group_max_f<LPP>(bub);
   const int bc = shfl_i<LPP>(bi, (int)__builtin_ctzll(group_ballot<LPP>(bub == gub) | (1ull << 63)));
+  // A group of two clusters' width sweeps the cluster with the second largest
+  // bound in step 2 as well, on its upper half (idle otherwise): the threshold
+  // is then the maximum over both clusters (it can only rise), and step 3 has
+  // one cluster fewer to visit.
+  constexpr bool kPair = LPP == 2 * kCluster;
+  int bc2 = -1;
+  if constexpr (kPair) {
+    float sub = -INFINITY;
+    int si = 0;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      const int c = lane + j * LPP;
+      if (c != bc && ub[j] > sub) {
+        sub = ub[j];
+        si = c;
+      }
+    }
+    for (int c = lane + K * LPP; c < nc; c += LPP) {
+      const float u = c != bc ? cluster_ub(cb[c], dx, dy, dz, dn) : -INFINITY;
+      if (u > sub) {
+        sub = u;
+        si = c;
+      }
+    }
+    const float g2 = group_max_f<LPP>(sub);
+    if (g2 > -INFINITY)
+      bc2 = shfl_i<LPP>(si, (int)__builtin_ctzll(group_ballot<LPP>(sub == g2) | (1ull << 63)));
+  }
+  // the slot of cluster bc (lower half) or bc2 (upper half) this lane sweeps
+  const int k2 = kPair ? ((lane < kCluster ? bc : bc2) * kCluster + (lane & (kCluster - 1))) : -1;
+  const bool has2 = kPair && (lane < kCluster || bc2 >= 0) && k2 < A.n;
 
   auto sweep = [&](int k) {
     const float px = sx[k], py = sy[k], pz = sz[k];
@@ -512,11 +543,16 @@ OGJK_DEV Sup support_clustered1(const CBody& A, const Query& q, int lane) {
       sw.bz = pz;
     }
   };
-  // 2: the best cluster: its FP32 maximum m0 gives the threshold L
+  // 2: the best cluster (and the second, see above): the FP32 maximum m0 of
+  // what was swept gives the threshold L
+  if constexpr (kPair) {
+    if (has2) sweep(k2);
+  } else {
 #pragma unroll
-  for (int t = lane; t < kCluster; t += LPP) {
-    const int k = bc * kCluster + t;
-    if (k < A.n) sweep(k);
+    for (int t = lane; t < kCluster; t += LPP) {
+      const int k = bc * kCluster + t;
+      if (k < A.n) sweep(k);
+    }
   }
   const float L = group_max_f<LPP>(sw.best) - 0x1.0p-15f * A.r1;
 
@@ -526,11 +562,13 @@ OGJK_DEV Sup support_clustered1(const CBody& A, const Query& q, int lane) {
 #pragma unroll
     for (int j = 0; j < K; j++) {
       const int c = lane + j * LPP;
-      visit_mask<LPP>(A, j * LPP, group_ballot<LPP>(c < nc && c != bc && !(ub[j] < L)), lane, f);
+      visit_mask<LPP>(A, j * LPP,
+                      group_ballot<LPP>(c < nc && c != bc && c != bc2 && !(ub[j] < L)), lane, f);
     }
     for (int base = K * LPP; base < nc; base += LPP) {
       const int c = base + lane;
-      const bool live = c < nc && c != bc && !(cluster_ub(cb[c], dx, dy, dz, dn) < L);
+      const bool live =
+          c < nc && c != bc && c != bc2 && !(cluster_ub(cb[c], dx, dy, dz, dn) < L);
       visit_mask<LPP>(A, base, group_ballot<LPP>(live), lane, f);
     }
   };
@@ -565,10 +603,14 @@ OGJK_DEV Sup support_clustered1(const CBody& A, const Query& q, int lane) {
         }
       }
     };
+    if constexpr (kPair) {
+      if (has2) rank(k2);
+    } else {
 #pragma unroll
-    for (int t = lane; t < kCluster; t += LPP) {
-      const int k = bc * kCluster + t;
-      if (k < A.n) rank(k);
+      for (int t = lane; t < kCluster; t += LPP) {
+        const int k = bc * kCluster + t;
+        if (k < A.n) rank(k);
+      }
     }
     others(rank);
   }
