@@ -426,8 +426,8 @@ def large_roofline(rec: dict, name: str) -> None:
 
 def small_extras(rec: dict, batch, args) -> None:
     """The 1,000-pair GJK configuration is the closest to the reference's
-    published figures: also time it end to end through the drop-in (device
-    allocation, host staging, copies, kernel, copies back) and set it beside
+    published figures: also time it end to end through the drop-in (host
+    staging, upload, kernel, download) and set it beside
     the reference's published kernel time."""
     import ogjk
     lg, le = WORKLOADS["sphere32_gjk"]["lanes"]
@@ -440,7 +440,8 @@ def small_extras(rec: dict, batch, args) -> None:
     el = (time.perf_counter() - t) / reps
     rec["end_to_end_dropin"] = {"ms_per_call": el * 1e3, "pairs_per_s": batch.n_pairs / el,
                                 "call": "ogjk_collide_host (compute_minimum_distance's path: "
-                                        "staging, hipMalloc, copies, kernel, copies back)"}
+                                        "host staging into kept pinned memory, one upload, "
+                                        "kernel, one download)"}
     rec["published_reference"] = {
         "kernel_ms": 0.0917, "pairs": 1000, "vertices": 50, "precision": "FP32",
         "gpu": "RTX 4070 (data file name)", "source": "data/data_32bit_4070:2",

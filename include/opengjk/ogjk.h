@@ -400,7 +400,10 @@ ogjk_status ogjk_collide_polytopes_indexed(int mode, const gkPolytope* d_pool,
 
 /* Host arrays in, host arrays out, synchronous; device memory is staged
  * internally.  Replaces compute_minimum_distance / computeCollisionInformation
- * / compute_gjk_epa (GJK/gpu/openGJK.h:91-141).  normals may be NULL for GJK. */
+ * / compute_gjk_epa (GJK/gpu/openGJK.h:91-141).  normals may be NULL for GJK.
+ * The staging buffers (device and pinned host, up to 256 MiB each per device)
+ * are kept between calls until the process ends; larger calls allocate and
+ * free their own.  Calls on one device are serialised. */
 ogjk_status ogjk_collide_host(int mode, int32_t n, const gkPolytope* bd1, const gkPolytope* bd2,
                               gkSimplex* simplices, float* distances, float* normals);
 

@@ -1,6 +1,7 @@
 // opengjk-amd C ABI: argument checking, host staging and launches.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -77,41 +78,93 @@ ogjk_status cached_workspace(size_t need, void** out) {
   return OGJK_OK;
 }
 
-// One device allocation carved into 256-byte aligned pieces, freed on scope exit.
+// One device allocation, freed on scope exit.
 struct DeviceArena {
   char* base = nullptr;
-  size_t used = 0, cap = 0;
   ~DeviceArena() {
     if (base) (void)hipFree(base);
   }
   static size_t up(size_t b) { return (b + 255) & ~size_t(255); }
-  hipError_t reserve(size_t bytes) {
-    cap = bytes;
-    return bytes ? hipMalloc((void**)&base, bytes) : hipSuccess;
-  }
-  template <class T>
-  T* take(size_t count) {
-    T* p = (T*)(base + used);
-    used += up(count * sizeof(T));
-    return p;
+  hipError_t reserve(size_t bytes) { return bytes ? hipMalloc((void**)&base, bytes) : hipSuccess; }
+};
+
+// Host staging for the host entry points.  Inputs are laid out in one
+// buffer exactly as they will sit on the device, so a call makes one upload
+// and one download:
+//   x | y | z | off | cnt | r1 | pairs | simplices | distances | normals | workspace
+// (each piece 256-byte aligned).  Upload: everything before the simplices, or
+// through the distances for EPA, which reads them.  Download: simplices
+// through distances, or through normals when they are asked for.  The
+// workspace lives on the device only.
+struct StageLayout {
+  size_t x, y, z, off, cnt, r1, pairs, simp, dist, nrm, ws, host_end, total;
+  StageLayout(size_t nv, size_t nb, size_t n, bool indexed) {
+    size_t o = 0;
+    auto put = [&](size_t bytes) {
+      const size_t at = o;
+      o += DeviceArena::up(bytes);
+      return at;
+    };
+    x = put(nv * 4);
+    y = put(nv * 4);
+    z = put(nv * 4);
+    off = put(nb * 8);
+    cnt = put(nb * 4);
+    r1 = put(nb * 4);
+    pairs = put(indexed ? n * 8 : 0);
+    simp = put(n * sizeof(gkSimplex));
+    dist = put(n * 4);
+    nrm = put(n * 12);
+    host_end = o;
+    ws = put(ogjk::workspace_bytes((int)n));
+    total = o;
   }
 };
 
-// Host SoA staging of a list of gkPolytope, with per-body L1 bound.
-struct HostScene {
-  std::vector<float> x, y, z, r1;
-  std::vector<int64_t> off;
-  std::vector<int32_t> cnt;
+// Per-device staging kept between calls: a device buffer and a pinned host
+// buffer, grown on demand up to kStageCacheMax bytes each.  A call that needs
+// more allocates its own device buffer and a pageable host buffer and frees
+// them on return.  The mutex serialises the host entry points on a device.
+constexpr size_t kStageCacheMax = size_t(256) << 20;
+struct StageCache {
+  std::mutex mu;
+  char* dev = nullptr;
+  char* host = nullptr;
+  size_t dev_bytes = 0, host_bytes = 0;
 };
+StageCache g_stage[64];
 
-ogjk_status stage_bodies(const gkPolytope* const* lists, const int* counts, int nlists,
-                         bool interleave, HostScene& hs) {
-  // interleave: body 2i = lists[0][i], 2i+1 = lists[1][i] (implicit pairs)
+ogjk_status stage_grow(StageCache& c, const StageLayout& L) {
+  const auto sized = [](size_t need) {
+    return std::min(std::max(need + need / 4, size_t(1) << 20), kStageCacheMax);
+  };
+  if (c.dev_bytes < L.total) {
+    if (c.dev) OGJK_HIP(hipFree(c.dev), "hipFree(staging)");
+    c.dev = nullptr;
+    c.dev_bytes = 0;
+    const size_t b = sized(L.total);
+    OGJK_HIP(hipMalloc((void**)&c.dev, b), "hipMalloc(staging)");
+    c.dev_bytes = b;
+  }
+  if (c.host_bytes < L.host_end) {
+    if (c.host) OGJK_HIP(hipHostFree(c.host), "hipHostFree(staging)");
+    c.host = nullptr;
+    c.host_bytes = 0;
+    const size_t b = sized(L.host_end);
+    OGJK_HIP(hipHostMalloc((void**)&c.host, b, hipHostMallocDefault), "hipHostMalloc(staging)");
+    c.host_bytes = b;
+  }
+  return OGJK_OK;
+}
+
+// Stage `lists` (interleave: body 2i = lists[0][i], 2i+1 = lists[1][i], the
+// implicit pairs; otherwise one pool addressed by `pairs`), run `mode` over
+// `n` pairs and copy the results back.  Synchronous.
+ogjk_status run_host(int mode, const gkPolytope* const* lists, const int* counts, int nlists,
+                     bool interleave, const gkCollisionPair* pairs, int n, gkSimplex* simplices,
+                     float* distances, float* normals) {
   int64_t nb = 0, nv = 0;
   for (int l = 0; l < nlists; l++) nb += counts[l];
-  hs.off.resize(nb);
-  hs.cnt.resize(nb);
-  hs.r1.resize(nb);
   auto body_at = [&](int64_t b) -> const gkPolytope& {
     if (interleave) return lists[b & 1][b >> 1];
     return lists[0][b];
@@ -121,72 +174,74 @@ ogjk_status stage_bodies(const gkPolytope* const* lists, const int* counts, int 
     if (p.numpoints < 0) return fail(OGJK_ERR_INVALID, "body %lld: numpoints %d < 0", (long long)b, p.numpoints);
     if (p.numpoints > 0 && p.coord == nullptr)
       return fail(OGJK_ERR_INVALID, "body %lld: coord is NULL", (long long)b);
-    hs.off[b] = nv;
-    hs.cnt[b] = p.numpoints;
     nv += p.numpoints;
   }
-  hs.x.resize(nv);
-  hs.y.resize(nv);
-  hs.z.resize(nv);
+  const StageLayout L((size_t)nv, (size_t)nb, (size_t)n, pairs != nullptr);
+  int dev = 0;
+  OGJK_HIP(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= 64) return fail(OGJK_ERR_UNSUPPORTED, "device id %d", dev);
+  StageCache& c = g_stage[dev];
+  std::lock_guard<std::mutex> lock(c.mu);
+  char *hb, *db;
+  std::vector<char> hvec;
+  DeviceArena big;
+  if (L.total <= kStageCacheMax) {
+    const ogjk_status s = stage_grow(c, L);
+    if (s != OGJK_OK) return s;
+    hb = c.host;
+    db = c.dev;
+  } else {
+    hvec.resize(L.host_end);
+    OGJK_HIP(big.reserve(L.total), "hipMalloc(staging)");
+    hb = hvec.data();
+    db = big.base;
+  }
+
+  float* hx = (float*)(hb + L.x);
+  float* hy = (float*)(hb + L.y);
+  float* hz = (float*)(hb + L.z);
+  int64_t* hoff = (int64_t*)(hb + L.off);
+  int32_t* hcnt = (int32_t*)(hb + L.cnt);
+  float* hr1 = (float*)(hb + L.r1);
+  int64_t o = 0;
   for (int64_t b = 0; b < nb; b++) {
     const gkPolytope& p = body_at(b);
-    const int64_t o = hs.off[b];
     float m = 0.0f;
     for (int i = 0; i < p.numpoints; i++) {
       const float vx = p.coord[3 * i], vy = p.coord[3 * i + 1], vz = p.coord[3 * i + 2];
-      hs.x[o + i] = vx;
-      hs.y[o + i] = vy;
-      hs.z[o + i] = vz;
+      hx[o + i] = vx;
+      hy[o + i] = vy;
+      hz[o + i] = vz;
       m = std::fmax(m, std::fabs(vx) + std::fabs(vy) + std::fabs(vz));
     }
-    hs.r1[b] = m * (1.0f + 0x1.0p-22f);
+    hoff[b] = o;
+    hcnt[b] = p.numpoints;
+    hr1[b] = m * (1.0f + 0x1.0p-22f);  // per-body L1 bound, rounded up
+    o += p.numpoints;
   }
-  return OGJK_OK;
-}
-
-// Upload a staged scene, run `mode` over `n` pairs and copy results back.
-ogjk_status run_staged(int mode, const HostScene& hs, const gkCollisionPair* pairs, int n,
-                       gkSimplex* simplices, float* distances, float* normals) {
-  const size_t nv = hs.x.size(), nb = hs.cnt.size();
-  DeviceArena ar;
-  size_t bytes = 3 * DeviceArena::up(nv * sizeof(float)) + DeviceArena::up(nb * 8) +
-                 2 * DeviceArena::up(nb * 4) + DeviceArena::up((size_t)n * sizeof(gkSimplex)) +
-                 DeviceArena::up((size_t)n * 4) + DeviceArena::up((size_t)n * 12) +
-                 DeviceArena::up((size_t)n * 8) + DeviceArena::up(ogjk::workspace_bytes(n)) + 256;
-  OGJK_HIP(ar.reserve(bytes), "hipMalloc(staging)");
-  float* dx = ar.take<float>(nv);
-  float* dy = ar.take<float>(nv);
-  float* dz = ar.take<float>(nv);
-  int64_t* doff = ar.take<int64_t>(nb);
-  int32_t* dcnt = ar.take<int32_t>(nb);
-  float* dr1 = ar.take<float>(nb);
-  gkSimplex* ds = ar.take<gkSimplex>(n);
-  float* dd = ar.take<float>(n);
-  float* dn = ar.take<float>(3 * (size_t)n);
-  int32_t* dp = pairs ? ar.take<int32_t>(2 * (size_t)n) : nullptr;
-  void* dws = ar.take<char>(ogjk::workspace_bytes(n));
-
-  OGJK_HIP(hipMemcpy(dx, hs.x.data(), nv * 4, hipMemcpyHostToDevice), "hipMemcpy(x)");
-  OGJK_HIP(hipMemcpy(dy, hs.y.data(), nv * 4, hipMemcpyHostToDevice), "hipMemcpy(y)");
-  OGJK_HIP(hipMemcpy(dz, hs.z.data(), nv * 4, hipMemcpyHostToDevice), "hipMemcpy(z)");
-  OGJK_HIP(hipMemcpy(doff, hs.off.data(), nb * 8, hipMemcpyHostToDevice), "hipMemcpy(off)");
-  OGJK_HIP(hipMemcpy(dcnt, hs.cnt.data(), nb * 4, hipMemcpyHostToDevice), "hipMemcpy(cnt)");
-  OGJK_HIP(hipMemcpy(dr1, hs.r1.data(), nb * 4, hipMemcpyHostToDevice), "hipMemcpy(r1)");
-  if (pairs) OGJK_HIP(hipMemcpy(dp, pairs, (size_t)n * 8, hipMemcpyHostToDevice), "hipMemcpy(pairs)");
+  if (pairs) std::memcpy(hb + L.pairs, pairs, (size_t)n * 8);
+  size_t up = L.simp;
   if (mode == OGJK_MODE_EPA) {
-    OGJK_HIP(hipMemcpy(ds, simplices, (size_t)n * sizeof(gkSimplex), hipMemcpyHostToDevice),
-             "hipMemcpy(simplices)");
-    OGJK_HIP(hipMemcpy(dd, distances, (size_t)n * 4, hipMemcpyHostToDevice), "hipMemcpy(distances)");
+    std::memcpy(hb + L.simp, simplices, (size_t)n * sizeof(gkSimplex));
+    std::memcpy(hb + L.dist, distances, (size_t)n * 4);
+    up = L.dist + (size_t)n * 4;
   }
-  const ogjk::SceneArgs sa{dx, dy, dz, doff, dcnt, dr1, (int32_t)nb, dp};
-  const ogjk::OutArgs oa{ds, dd, dn, nullptr};
-  OGJK_HIP(ogjk::launch_scene(mode, sa, oa, n, dws, nullptr), "kernel launch");
-  OGJK_HIP(hipDeviceSynchronize(), "kernel");
-  OGJK_HIP(hipMemcpy(simplices, ds, (size_t)n * sizeof(gkSimplex), hipMemcpyDeviceToHost),
-           "hipMemcpy(simplices)");
-  OGJK_HIP(hipMemcpy(distances, dd, (size_t)n * 4, hipMemcpyDeviceToHost), "hipMemcpy(distances)");
-  if ((mode & OGJK_MODE_EPA) && normals)
-    OGJK_HIP(hipMemcpy(normals, dn, (size_t)n * 12, hipMemcpyDeviceToHost), "hipMemcpy(normals)");
+  const bool want_n = (mode & OGJK_MODE_EPA) && normals;
+  const size_t down = (want_n ? L.nrm + (size_t)n * 12 : L.dist + (size_t)n * 4) - L.simp;
+
+  OGJK_HIP(hipMemcpyAsync(db, hb, up, hipMemcpyHostToDevice, nullptr), "hipMemcpy(upload)");
+  const ogjk::SceneArgs sa{(float*)(db + L.x), (float*)(db + L.y), (float*)(db + L.z),
+                           (int64_t*)(db + L.off), (int32_t*)(db + L.cnt), (float*)(db + L.r1),
+                           (int32_t)nb, pairs ? (int32_t*)(db + L.pairs) : nullptr};
+  const ogjk::OutArgs oa{(gkSimplex*)(db + L.simp), (float*)(db + L.dist), (float*)(db + L.nrm),
+                         nullptr};
+  OGJK_HIP(ogjk::launch_scene(mode, sa, oa, n, db + L.ws, nullptr), "kernel launch");
+  OGJK_HIP(hipMemcpyAsync(hb + L.simp, db + L.simp, down, hipMemcpyDeviceToHost, nullptr),
+           "hipMemcpy(download)");
+  OGJK_HIP(hipStreamSynchronize(nullptr), "kernel");
+  std::memcpy(simplices, hb + L.simp, (size_t)n * sizeof(gkSimplex));
+  std::memcpy(distances, hb + L.dist, (size_t)n * 4);
+  if (want_n) std::memcpy(normals, hb + L.nrm, (size_t)n * 12);
   return OGJK_OK;
 }
 
@@ -640,10 +695,7 @@ ogjk_status ogjk_collide_host(int mode, int32_t n, const gkPolytope* bd1, const 
   if (!bd1 || !bd2) return fail(OGJK_ERR_INVALID, "NULL polytope array");
   const gkPolytope* lists[2] = {bd1, bd2};
   const int counts[2] = {n, n};
-  HostScene hs;
-  const ogjk_status s2 = stage_bodies(lists, counts, 2, true, hs);
-  if (s2 != OGJK_OK) return s2;
-  return run_staged(mode, hs, nullptr, n, simplices, distances, normals);
+  return run_host(mode, lists, counts, 2, true, nullptr, n, simplices, distances, normals);
 }
 
 ogjk_status ogjk_collide_host_indexed(int mode, int32_t n_polytopes, int32_t n_pairs,
@@ -656,10 +708,7 @@ ogjk_status ogjk_collide_host_indexed(int mode, int32_t n_polytopes, int32_t n_p
     return fail(OGJK_ERR_INVALID, "NULL polytopes/pairs or empty pool");
   const gkPolytope* lists[1] = {polytopes};
   const int counts[1] = {n_polytopes};
-  HostScene hs;
-  const ogjk_status s2 = stage_bodies(lists, counts, 1, false, hs);
-  if (s2 != OGJK_OK) return s2;
-  return run_staged(mode, hs, pairs, n_pairs, simplices, distances, normals);
+  return run_host(mode, lists, counts, 1, false, pairs, n_pairs, simplices, distances, normals);
 }
 
 }  // extern "C"

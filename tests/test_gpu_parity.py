@@ -301,6 +301,42 @@ def test_host_paths_agree(torch_cuda):
     assert np.array_equal(ni.view(np.uint32), n1.view(np.uint32))
 
 
+def test_host_staging_reuse(torch_cuda):
+    """The host entry points keep their staging buffers between calls: calls
+    that grow, shrink and grow again, in every mode, each against the oracle,
+    and a call too large for the kept buffers (its own allocations) against
+    the device-resident path on the same pairs."""
+    torch = torch_cuda
+    small = gen.make(gen.SPHERE, 300, seed=61, nverts=32, box=1.2)
+    mid = gen.make(gen.ELLIPSOID, 6000, seed=62, nmin=8, nmax=256, box=1.5)
+    refs = {}
+    for b in (small, mid, small, mid):
+        if id(b) not in refs:
+            refs[id(b)] = oracle_run(oracle.GJK_EPA, b)
+        ref = refs[id(b)]
+        s, d, n = ogjk.collide_host(oracle.GJK_EPA, b)
+        r = compare_f32(s, d, n, ref, oracle.GJK_EPA)
+        assert r["exact"] == r["n"], r
+        sg, dg, _ = ogjk.collide_host(oracle.GJK, b)
+        se, de, ne = ogjk.collide_host(oracle.EPA, b, simplices=sg, distances=dg)
+        sep = ref.distances > 0  # GJK alone gives these pairs' final distances
+        assert np.array_equal(dg[sep].view(np.uint32), d[sep].view(np.uint32))
+        r = compare_f32(se, de, ne, ref, oracle.GJK_EPA)
+        assert r["max_dist_rel"] <= 1e-5, r
+    # 100,000 pairs of 128-vertex bodies: 307 MB of coordinates, above the
+    # 256 MiB kept per device
+    big = gen.make(gen.SPHERE, 100_000, seed=63, nverts=128, box=3.0)
+    s, d, n = ogjk.collide_host(oracle.GJK_EPA, big)
+    sc = ogjk.DeviceScene(big)
+    pairs = torch.from_numpy(np.stack([big.pa, big.pb], 1).astype(np.int32)).cuda()
+    dev = sc.collide(oracle.GJK_EPA, big.n_pairs, pairs=pairs)
+    torch.cuda.synchronize()
+    assert np.array_equal(d.view(np.uint32), dev["distances"].cpu().numpy().view(np.uint32))
+    assert np.array_equal(n.view(np.uint32), dev["normals"].cpu().numpy().view(np.uint32))
+    assert np.array_equal(s.view(np.uint8), ogjk.simplices_from_device(dev["simplices"]).view(np.uint8))
+    assert (d < 0).any() and (d > 0).any()
+
+
 def test_invalid_pairs_flagged(torch_cuda):
     b = gen.make(gen.SPHERE, 8, seed=9, nverts=16, box=1.0)
     pairs = np.array([[0, 1], [2, 99], [-1, 3], [4, 5]], np.int32)
