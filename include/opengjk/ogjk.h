@@ -56,11 +56,14 @@ typedef struct ogjk_clusters {
   float* y;
   float* z;
   int32_t* orig;       /* n_slots: body-local input index of each slot (-1: padding) */
-  int64_t* body_slot;  /* n_bodies: first slot of each body (a multiple of 32) */
+  int64_t* body_slot;  /* n_bodies: first slot of each body (a multiple of 32), or
+                          -1: not built (fewer than min_verts vertices at the build) */
   float* bounds;       /* 8 * n_slots / 32: per cluster centre x, y, z, radius,
                           half extents x, y, z, unused */
   int64_t n_slots;     /* capacity of x, y, z, orig: >= ogjk_cluster_slots() */
-  int32_t min_verts;   /* smaller bodies are swept without culling */
+  int32_t min_verts;   /* smaller bodies are swept without culling; the build
+                          clusters only bodies of at least min_verts vertices, so
+                          lowering it afterwards culls no further bodies */
   float* super_bounds; /* 8 * n_supers: bounds of each run of 32 clusters of the
                           bodies of more than 4,096 vertices, as bounds */
   int64_t* body_super; /* n_bodies: first supercluster of each body, -1: none */
@@ -365,7 +368,8 @@ int64_t ogjk_cluster_slots(int64_t n_verts, int32_t n_bodies);
 int64_t ogjk_cluster_supers(int64_t n_slots);
 
 /* Builds the clusters of every body of a packed scene (scene->x/y/z,
- * body_off, body_cnt) into caller-allocated arrays.  Synchronous on stream;
+ * body_off, body_cnt) with at least clusters->min_verts vertices into
+ * caller-allocated arrays.  Synchronous on stream;
  * allocates scratch for the per-body sort.  No counterpart in the reference:
  * an acceleration structure for the support function
  * (GJK/cpu/openGJK.c:615-639, GJK/cpu/EPA.c:307-344) on large hulls. */

@@ -586,7 +586,7 @@ ogjk_status ogjk_build_clusters(const ogjk_scene* scene, ogjk_clusters* cl, void
   const hipError_t e = ogjk::build_clusters(
       scene->x, scene->y, scene->z, scene->body_off, scene->body_cnt, scene->n_bodies, cl->x,
       cl->y, cl->z, cl->orig, cl->body_slot, cl->bounds, cl->n_slots, cl->super_bounds,
-      cl->body_super, cl->n_supers, &cl->n_supers_used, (hipStream_t)stream);
+      cl->body_super, cl->n_supers, &cl->n_supers_used, cl->min_verts, (hipStream_t)stream);
   if (e == hipErrorInvalidValue)
     return fail(OGJK_ERR_INVALID, "clusters need more than %lld slots or %lld superclusters, or "
                 "a body exceeds 2^27 vertices", (long long)cl->n_slots, (long long)cl->n_supers);

@@ -26,31 +26,68 @@ OGJK_DEV int body_class(int n) {
   return n <= 64 ? 0 : (n <= 256 ? 1 : (n <= 1024 ? 2 : (n <= 2048 ? 3 : 4)));
 }
 
-// Per-class lists of the bodies built by one block each (0 < n <= lds_max):
-// lists[k * n_bodies + i].  One atomic per class per block; the order inside
-// a list is arbitrary (a body's output depends only on its own slots).
+// Per body: its padded slot count when it is built (n >= min_build), else 0,
+// with psize[n_bodies] = 0 closing the scan; and its list: one of the
+// one-block classes (n <= lds_max) or kClasses, the batched bodies, as
+// lists[k * n_bodies + i].  One atomic per list per block; the order inside a
+// list is arbitrary (a body's output depends only on its own slots).
 __global__ void __launch_bounds__(256) k_body_classes(const int32_t* __restrict__ cnt,
-                                                      int n_bodies, int lds_max,
+                                                      int n_bodies, int min_build, int lds_max,
                                                       unsigned* __restrict__ counts,
-                                                      int* __restrict__ lists) {
-  __shared__ unsigned hist[kClasses], base[kClasses];
-  if (threadIdx.x < kClasses) hist[threadIdx.x] = 0u;
+                                                      int* __restrict__ lists,
+                                                      int64_t* __restrict__ psize) {
+  __shared__ unsigned hist[kClasses + 1], base[kClasses + 1];
+  if (threadIdx.x <= kClasses) hist[threadIdx.x] = 0u;
   __syncthreads();
   const int b = blockIdx.x * 256 + threadIdx.x;
   int k = -1;
   unsigned r = 0;
   if (b < n_bodies) {
     const int n = cnt[b];
-    if (n > 0 && n <= lds_max) {
-      k = body_class(n);
+    const bool built = n > 0 && n >= min_build;
+    psize[b] = built ? ((int64_t)n + kCluster - 1) / kCluster * kCluster : 0;
+    if (built) {
+      k = n <= lds_max ? body_class(n) : kClasses;
       r = atomicAdd(&hist[k], 1u);
     }
+  } else if (b == n_bodies) {
+    psize[b] = 0;
   }
   __syncthreads();
-  if (threadIdx.x < kClasses)
+  if (threadIdx.x <= kClasses)
     base[threadIdx.x] = hist[threadIdx.x] ? atomicAdd(&counts[threadIdx.x], hist[threadIdx.x]) : 0u;
   __syncthreads();
   if (k >= 0) lists[(size_t)k * n_bodies + base[k] + r] = b;
+}
+
+// body_slot[b]: first slot of a built body (the exclusive scan of psize), -1
+// for the others; body_super[b] = -1 (k_set_super marks the superclustered).
+__global__ void __launch_bounds__(256) k_body_slots(const int64_t* __restrict__ psize,
+                                                    const int64_t* __restrict__ pos, int n_bodies,
+                                                    int64_t* __restrict__ body_slot,
+                                                    int64_t* __restrict__ body_super) {
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= n_bodies) return;
+  body_slot[b] = psize[b] > 0 ? pos[b] : -1;
+  body_super[b] = -1;
+}
+
+// Count and first slot of each batched body: out[2q], out[2q + 1].
+__global__ void __launch_bounds__(256) k_gather_large(const int32_t* __restrict__ cnt,
+                                                      const int64_t* __restrict__ body_slot,
+                                                      const int* __restrict__ large, int n_large,
+                                                      int64_t* __restrict__ out) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= n_large) return;
+  out[2 * q] = cnt[large[q]];
+  out[2 * q + 1] = body_slot[large[q]];
+}
+
+// body_super[bs[2q]] = bs[2q + 1]: first supercluster of each superclustered body.
+__global__ void __launch_bounds__(256) k_set_super(const int64_t* __restrict__ bs, int n,
+                                                   int64_t* __restrict__ body_super) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q < n) body_super[bs[2 * q]] = bs[2 * q + 1];
 }
 
 // Sort keys: Hilbert index on a 2^kGridBits x 2^kGridBits grid over the
@@ -444,7 +481,7 @@ __global__ void __launch_bounds__(256) k_mark_super(const int32_t* __restrict__ 
                                                     const int64_t* __restrict__ super,
                                                     int n_bodies, ClusterBound* __restrict__ b) {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n_bodies || cnt[i] <= 0) return;
+  if (i >= n_bodies || cnt[i] <= 0 || slot[i] < 0) return;
   b[slot[i] / kCluster].h.w = __builtin_bit_cast(float, (int)super[i]);
 }
 
@@ -460,93 +497,33 @@ int64_t cluster_slots(int64_t n_verts, int32_t n_bodies) {
   return n_verts + (int64_t)(kCluster - 1) * (n_bodies > 0 ? n_bodies : 0);
 }
 
-// Builds the clusters of every body.  Bodies are processed in batches of at
-// most kBatch vertices so the sort's scratch stays bounded; body_slot is
-// computed on the host from the counts.
+// Builds the clusters of every body of at least min_build vertices; the
+// others get body_slot -1 and no slots (queries sweep them whole).  The slot
+// layout (a scan of the padded sizes) and the per-class body lists are made
+// on the device; only the batched bodies (more than lds_max vertices) come
+// back to the host, which cuts them into batches of at most kBatch vertices
+// so the sort's scratch stays bounded.
 hipError_t build_clusters(const float* x, const float* y, const float* z, const int64_t* off,
                           const int32_t* cnt, int32_t n_bodies, float* cx, float* cy, float* cz,
                           int32_t* orig, int64_t* body_slot, float* bounds, int64_t n_slots,
                           float* sbounds, int64_t* body_super, int64_t n_supers,
-                          int64_t* n_supers_used, hipStream_t st) {
+                          int64_t* n_supers_used, int32_t min_build, hipStream_t st) {
   constexpr int64_t kBatch = int64_t(1) << 27;
   if (n_bodies <= 0) return hipSuccess;
-  std::vector<int32_t> hcnt(n_bodies);
-  hipError_t e = hipMemcpyAsync(hcnt.data(), cnt, sizeof(int32_t) * n_bodies,
-                                hipMemcpyDeviceToHost, st);
-  if (e != hipSuccess || (e = hipStreamSynchronize(st)) != hipSuccess) return e;
-  std::vector<int64_t> hslot(n_bodies);
-  int64_t total = 0, maxn = 0;
-  for (int b = 0; b < n_bodies; b++) {
-    const int64_t n = hcnt[b] > 0 ? hcnt[b] : 0;
-    hslot[b] = total;
-    total += (n + kCluster - 1) / kCluster * kCluster;
-    maxn = std::max(maxn, n);
-  }
-  if (total > n_slots || maxn > kBatch) return hipErrorInvalidValue;
-  e = hipMemcpyAsync(body_slot, hslot.data(), sizeof(int64_t) * n_bodies, hipMemcpyHostToDevice,
-                     st);
-  if (e != hipSuccess) return e;
-  // superclusters of the bodies with more than kSuper clusters
-  std::vector<int64_t> hsuper(n_bodies, -1), sfirst;
-  std::vector<int32_t> scount;
-  for (int b = 0; b < n_bodies; b++) {
-    const int64_t n = hcnt[b] > 0 ? hcnt[b] : 0;
-    const int64_t nc = (n + kCluster - 1) / kCluster;
-    if (nc <= kSuperMinClusters) continue;
-    hsuper[b] = (int64_t)sfirst.size();
-    for (int64_t c0 = 0; c0 < nc; c0 += kSuper) {
-      sfirst.push_back(hslot[b] + c0 * kCluster);
-      scount.push_back((int32_t)(std::min<int64_t>(kSuper, nc - c0) * kCluster));
-    }
-  }
-  if ((int64_t)sfirst.size() > n_supers || (int64_t)sfirst.size() > 0x7fffffff)
-    return hipErrorInvalidValue;
-  *n_supers_used = (int64_t)sfirst.size();
-  e = sfirst.empty() ? hipMemsetAsync(body_super, 0xff, sizeof(int64_t) * n_bodies, st)  // all -1
-                     : hipMemcpyAsync(body_super, hsuper.data(), sizeof(int64_t) * n_bodies,
-                                      hipMemcpyHostToDevice, st);
-  if (e != hipSuccess) return e;
-
+  if (n_bodies == 0x7fffffff) return hipErrorInvalidValue;  // n_bodies + 1 scan entries
+  min_build = std::max(min_build, 1);
   // Bodies of at most lds_max vertices are built whole by one block each
-  // (k_cluster_body, three size classes); larger ones go through batches of
+  // (k_cluster_body, five size classes); larger ones go through batches of
   // keys, a device segmented sort and a scatter.  OGJK_CLUSTER_LDS_MAX=0 sends
   // every body through the batches (the two paths give identical arrays).
   int lds_max = kLdsBodyMax;
   if (const char* v = getenv("OGJK_CLUSTER_LDS_MAX"))
     lds_max = std::max(0, std::min(atoi(v), kLdsBodyMax));
-  std::vector<int> large;  // the batched bodies
-  for (int b = 0; b < n_bodies; b++)
-    if (hcnt[b] > lds_max) large.push_back(b);
-  // batches of consecutive large bodies (positions in `large`)
-  std::vector<int> bstart{0};
-  std::vector<int> kbeg_h;  // batch-local key offset of every large body, batch after batch
-  std::vector<int64_t> bverts;
-  {
-    int64_t acc = 0;
-    for (int q = 0; q < (int)large.size(); q++) {
-      const int64_t n = hcnt[large[q]];
-      if (acc + n > kBatch) {
-        bstart.push_back(q);
-        bverts.push_back(acc);
-        acc = 0;
-      }
-      kbeg_h.push_back((int)acc);
-      acc += n;
-    }
-    bverts.push_back(acc);
-    bstart.push_back((int)large.size());
-  }
-  int64_t maxv = 0;
-  int maxb = 0;
-  for (size_t i = 0; i + 1 < bstart.size(); i++) {
-    maxv = std::max(maxv, bverts[i]);
-    maxb = std::max(maxb, bstart[i + 1] - bstart[i]);
-  }
-  maxv = std::max<int64_t>(maxv, 1);
 
   uint32_t *keys = nullptr, *keys2 = nullptr;
   int *vals = nullptr, *vals2 = nullptr, *kbeg = nullptr, *blist = nullptr, *lbig = nullptr;
   unsigned* ccount = nullptr;
+  int64_t *psize = nullptr, *pos = nullptr, *dgath = nullptr;
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
   auto cleanup = [&]() {
@@ -558,26 +535,48 @@ hipError_t build_clusters(const float* x, const float* y, const float* z, const 
     (void)hipFree(blist);
     (void)hipFree(lbig);
     (void)hipFree(ccount);
+    (void)hipFree(psize);
+    (void)hipFree(pos);
+    (void)hipFree(dgath);
     (void)hipFree(tmp);
   };
-  // the one-block bodies: class lists built on the device
-  unsigned hcount[kClasses] = {};
-  if (lds_max > 0) {
-    if ((e = hipMalloc(&blist, sizeof(int) * (size_t)kClasses * n_bodies)) != hipSuccess ||
-        (e = hipMalloc(&ccount, sizeof(unsigned) * kClasses)) != hipSuccess ||
-        (e = hipMemsetAsync(ccount, 0, sizeof(unsigned) * kClasses, st)) != hipSuccess) {
-      cleanup();
-      return e;
-    }
-    hipLaunchKernelGGL(k_body_classes, dim3((unsigned)((n_bodies + 255) / 256)), dim3(256), 0, st,
-                       cnt, n_bodies, lds_max, ccount, blist);
-    if ((e = hipMemcpyAsync(hcount, ccount, sizeof(hcount), hipMemcpyDeviceToHost, st)) !=
-            hipSuccess ||
-        (e = hipStreamSynchronize(st)) != hipSuccess) {
-      cleanup();
-      return e;
-    }
-  }
+  auto fail_with = [&](hipError_t err) {
+    cleanup();
+    return err;
+  };
+  hipError_t e;
+
+  // slot layout and body lists
+  const int nb1 = n_bodies + 1;
+  size_t scan_bytes = 0;
+  if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, psize, pos, nb1, st)) != hipSuccess)
+    return fail_with(e);
+  tmp_bytes = std::max<size_t>(scan_bytes, 1);
+  if ((e = hipMalloc(&blist, sizeof(int) * (size_t)(kClasses + 1) * n_bodies)) != hipSuccess ||
+      (e = hipMalloc(&ccount, sizeof(unsigned) * (kClasses + 1))) != hipSuccess ||
+      (e = hipMalloc(&psize, sizeof(int64_t) * nb1)) != hipSuccess ||
+      (e = hipMalloc(&pos, sizeof(int64_t) * nb1)) != hipSuccess ||
+      (e = hipMalloc(&tmp, tmp_bytes)) != hipSuccess ||
+      (e = hipMemsetAsync(ccount, 0, sizeof(unsigned) * (kClasses + 1), st)) != hipSuccess)
+    return fail_with(e);
+  hipLaunchKernelGGL(k_body_classes, dim3((unsigned)((nb1 + 255) / 256)), dim3(256), 0, st, cnt,
+                     n_bodies, min_build, lds_max, ccount, blist, psize);
+  if ((e = hipcub::DeviceScan::ExclusiveSum(tmp, scan_bytes, psize, pos, nb1, st)) != hipSuccess)
+    return fail_with(e);
+  hipLaunchKernelGGL(k_body_slots, dim3((unsigned)((n_bodies + 255) / 256)), dim3(256), 0, st,
+                     psize, pos, n_bodies, body_slot, body_super);
+  unsigned hcount[kClasses + 1] = {};
+  int64_t total = 0;
+  if ((e = hipGetLastError()) != hipSuccess ||
+      (e = hipMemcpyAsync(hcount, ccount, sizeof(hcount), hipMemcpyDeviceToHost, st)) !=
+          hipSuccess ||
+      (e = hipMemcpyAsync(&total, pos + n_bodies, sizeof(int64_t), hipMemcpyDeviceToHost, st)) !=
+          hipSuccess ||
+      (e = hipStreamSynchronize(st)) != hipSuccess)
+    return fail_with(e);
+  if (total > n_slots) return fail_with(hipErrorInvalidValue);
+
+  // the one-block bodies
   {
     ClusterBound* cb = (ClusterBound*)bounds;
     auto launch = [&](auto kernel, int k, int block) {
@@ -590,21 +589,93 @@ hipError_t build_clusters(const float* x, const float* y, const float* z, const 
     launch(k_cluster_body<256, 4>, 2, 256);
     launch(k_cluster_body<256, 8>, 3, 256);  // 512-thread blocks measured slower
     launch(k_cluster_body<256, 16>, 4, 256);
-    if ((e = hipGetLastError()) != hipSuccess) {
-      cleanup();
-      return e;
+    if ((e = hipGetLastError()) != hipSuccess) return fail_with(e);
+  }
+
+  // the batched bodies, in index order (the supercluster numbering follows it)
+  const int n_large = (int)hcount[kClasses];
+  std::vector<int> large(n_large);
+  std::vector<int64_t> lcs(2 * (size_t)n_large);  // count, first slot
+  if (n_large > 0) {
+    if ((e = hipMemcpyAsync(large.data(), blist + (size_t)kClasses * n_bodies,
+                            sizeof(int) * n_large, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+        (e = hipStreamSynchronize(st)) != hipSuccess)
+      return fail_with(e);
+    std::sort(large.begin(), large.end());
+    if ((e = hipMalloc(&lbig, sizeof(int) * n_large)) != hipSuccess ||
+        (e = hipMalloc(&dgath, sizeof(int64_t) * 2 * n_large)) != hipSuccess ||
+        (e = hipMemcpyAsync(lbig, large.data(), sizeof(int) * n_large, hipMemcpyHostToDevice,
+                            st)) != hipSuccess)
+      return fail_with(e);
+    hipLaunchKernelGGL(k_gather_large, dim3((unsigned)((n_large + 255) / 256)), dim3(256), 0, st,
+                       cnt, body_slot, lbig, n_large, dgath);
+    if ((e = hipGetLastError()) != hipSuccess ||
+        (e = hipMemcpyAsync(lcs.data(), dgath, sizeof(int64_t) * 2 * n_large,
+                            hipMemcpyDeviceToHost, st)) != hipSuccess ||
+        (e = hipStreamSynchronize(st)) != hipSuccess)
+      return fail_with(e);
+    for (int q = 0; q < n_large; q++)
+      if (lcs[2 * q] > kBatch) return fail_with(hipErrorInvalidValue);
+  }
+  // superclusters of the bodies with more than kSuperMinClusters clusters:
+  // all batched (kSuperMinClusters * kCluster >= kLdsBodyMax >= lds_max)
+  static_assert(kSuperMinClusters * kCluster >= kLdsBodyMax, "superclustered bodies are batched");
+  std::vector<int64_t> sfirst, sbody;  // sbody: (body, first supercluster) pairs
+  std::vector<int32_t> scount;
+  for (int q = 0; q < n_large; q++) {
+    const int64_t nc = (lcs[2 * q] + kCluster - 1) / kCluster;
+    if (nc <= kSuperMinClusters) continue;
+    sbody.push_back(large[q]);
+    sbody.push_back((int64_t)sfirst.size());
+    for (int64_t c0 = 0; c0 < nc; c0 += kSuper) {
+      sfirst.push_back(lcs[2 * q + 1] + c0 * kCluster);
+      scount.push_back((int32_t)(std::min<int64_t>(kSuper, nc - c0) * kCluster));
     }
   }
+  if ((int64_t)sfirst.size() > n_supers || (int64_t)sfirst.size() > 0x7fffffff)
+    return fail_with(hipErrorInvalidValue);
+  *n_supers_used = (int64_t)sfirst.size();
+  if (!sbody.empty()) {
+    // dgath (2 * n_large entries) holds the sbody pairs: one per batched body at most
+    const int ns = (int)(sbody.size() / 2);
+    if ((e = hipMemcpyAsync(dgath, sbody.data(), sizeof(int64_t) * sbody.size(),
+                            hipMemcpyHostToDevice, st)) != hipSuccess)
+      return fail_with(e);
+    hipLaunchKernelGGL(k_set_super, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st, dgath,
+                       ns, body_super);
+  }
+
+  // batches of consecutive batched bodies (positions in `large`)
+  std::vector<int> bstart{0};
+  std::vector<int> kbeg_h;  // batch-local key offset of every batched body, batch after batch
+  std::vector<int64_t> bverts;
+  {
+    int64_t acc = 0;
+    for (int q = 0; q < n_large; q++) {
+      const int64_t n = lcs[2 * q];
+      if (acc + n > kBatch) {
+        bstart.push_back(q);
+        bverts.push_back(acc);
+        acc = 0;
+      }
+      kbeg_h.push_back((int)acc);
+      acc += n;
+    }
+    bverts.push_back(acc);
+    bstart.push_back(n_large);
+  }
+  int64_t maxv = 0;
+  int maxb = 0;
+  for (size_t i = 0; i + 1 < bstart.size(); i++) {
+    maxv = std::max(maxv, bverts[i]);
+    maxb = std::max(maxb, bstart[i + 1] - bstart[i]);
+  }
+  maxv = std::max<int64_t>(maxv, 1);
   if (!large.empty() &&
       ((e = hipMalloc(&keys, 4 * maxv)) != hipSuccess || (e = hipMalloc(&keys2, 4 * maxv)) ||
        (e = hipMalloc(&vals, 4 * maxv)) || (e = hipMalloc(&vals2, 4 * maxv)) ||
-       (e = hipMalloc(&kbeg, sizeof(int) * ((size_t)maxb + 1))) ||
-       (e = hipMalloc(&lbig, sizeof(int) * large.size())) ||
-       (e = hipMemcpyAsync(lbig, large.data(), sizeof(int) * large.size(), hipMemcpyHostToDevice,
-                           st)))) {
-    cleanup();
-    return e;
-  }
+       (e = hipMalloc(&kbeg, sizeof(int) * ((size_t)maxb + 1)))))
+    return fail_with(e);
   for (size_t i = 0; i + 1 < bstart.size() && !large.empty(); i++) {
     const int b0 = bstart[i], nb = bstart[i + 1] - b0;
     const int nv = (int)bverts[i];

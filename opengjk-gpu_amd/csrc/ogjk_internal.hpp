@@ -116,7 +116,7 @@ hipError_t build_clusters(const float* x, const float* y, const float* z, const 
                           const int32_t* cnt, int32_t n_bodies, float* cx, float* cy, float* cz,
                           int32_t* orig, int64_t* body_slot, float* bounds, int64_t n_slots,
                           float* sbounds, int64_t* body_super, int64_t n_supers,
-                          int64_t* n_supers_used, hipStream_t st);
+                          int64_t* n_supers_used, int32_t min_build, hipStream_t st);
 hipError_t launch_pack_f64_aos(const float* aos, int64_t n_verts, double* xyz, hipStream_t st);
 hipError_t launch_pack_f64(const float* aos, int64_t n_verts, double* x, double* y, double* z,
                            hipStream_t st);

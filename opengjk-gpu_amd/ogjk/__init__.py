@@ -289,8 +289,8 @@ class DeviceScene:
             self.build_clusters(cluster_min, stream)
 
     def build_clusters(self, min_verts: int = 512, stream=None):
-        """Build the vertex clusters (ogjk_build_clusters) and use them for
-        bodies of at least min_verts vertices."""
+        """Build the vertex clusters (ogjk_build_clusters) of the bodies of
+        at least min_verts vertices and use them for those bodies."""
         t = self.torch
         dev = self.x.device
         slots = int(lib().ogjk_cluster_slots(self.n_verts, self.n_bodies))
@@ -313,7 +313,9 @@ class DeviceScene:
         self.scene.clusters = C.pointer(self.clusters)
 
     def set_cluster_min(self, min_verts: int) -> None:
-        """Culling threshold in vertices (needs build_clusters)."""
+        """Culling threshold in vertices (needs build_clusters).  Bodies
+        below the threshold the clusters were built with have none, so a
+        lower value culls no further bodies."""
         if self.clusters is None:
             raise OgjkError("no clusters built")
         self.clusters.min_verts = int(min_verts)

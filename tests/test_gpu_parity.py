@@ -127,28 +127,32 @@ def test_cluster_layout(torch_cuda):
                 assert (np.abs(q - ctr) <= h).all()
 
 
-def _cluster_arrays(torch, batch, env):
-    """Builds the scene's clusters with OGJK_CLUSTER_LDS_MAX=env (None: unset)
-    and returns every array the build writes, cut to the filled part."""
+def _cluster_arrays(torch, batch, env, cmin=1):
+    """Builds the scene's clusters of the bodies of at least cmin vertices
+    with OGJK_CLUSTER_LDS_MAX=env (None: unset) and returns every array the
+    build writes, cut to the filled part."""
     import os
     old = os.environ.pop("OGJK_CLUSTER_LDS_MAX", None)
     try:
         if env is not None:
             os.environ["OGJK_CLUSTER_LDS_MAX"] = env
         sc = ogjk.DeviceScene(batch)
-        sc.build_clusters(1)
+        sc.build_clusters(cmin)
         torch.cuda.synchronize()
     finally:
         os.environ.pop("OGJK_CLUSTER_LDS_MAX", None)
         if old is not None:
             os.environ["OGJK_CLUSTER_LDS_MAX"] = old
     cnt = batch.cnt.astype(np.int64)
-    total = int((-(-cnt // 32) * 32).sum())
+    built = cnt >= max(cmin, 1)
+    total = int((-(-cnt[built] // 32) * 32).sum())
+    slot = sc.cslot.cpu().numpy()
+    assert (slot[~built] == -1).all() and (slot[built] >= 0).all()
     ns = int(sc.clusters.n_supers_used)
     bits = lambda t, n: t.cpu().numpy()[:n].view(np.uint32)  # noqa: E731
     return {"x": bits(sc.cx, total), "y": bits(sc.cy, total), "z": bits(sc.cz, total),
             "orig": bits(sc.corig, total), "bounds": bits(sc.cbounds, total // 32 * 8),
-            "slot": sc.cslot.cpu().numpy(), "super": sc.csuper.cpu().numpy(),
+            "slot": slot, "super": sc.csuper.cpu().numpy(),
             "sbounds": bits(sc.csbounds, ns * 8), "n_supers": ns}
 
 
@@ -173,11 +177,32 @@ def test_cluster_build_paths_identical(torch_cuda):
     mixed = gen.from_bodies(A, B)
     small = gen.make(gen.ELLIPSOID, 300, seed=7, box=1.5, nmin=4, nmax=4096)
     for name, batch in (("mixed sizes", mixed), ("ellipsoids <= 4096", small)):
-        ref = _cluster_arrays(torch_cuda, batch, "0")
-        new = _cluster_arrays(torch_cuda, batch, None)
-        assert ref["n_supers"] == new["n_supers"], name
-        for k in ref:
-            assert np.array_equal(ref[k], new[k]), f"{name}: {k} differs"
+        for cmin in (1, 300):  # 300: the smaller bodies are not built
+            ref = _cluster_arrays(torch_cuda, batch, "0", cmin)
+            new = _cluster_arrays(torch_cuda, batch, None, cmin)
+            assert ref["n_supers"] == new["n_supers"], name
+            for k in ref:
+                assert np.array_equal(ref[k], new[k]), f"{name}, from {cmin}: {k} differs"
+
+
+def test_cluster_threshold_lowered_after_build(torch_cuda):
+    """Clusters built from 512 vertices, then the culling threshold lowered
+    to 1: the smaller bodies have no clusters and are swept whole; results
+    stay exact."""
+    torch = torch_cuda
+    b = gen.make(gen.ELLIPSOID, 1500, seed=13, box=1.5, nmin=4, nmax=3000)
+    ref = oracle_run(oracle.GJK_EPA, b)
+    sc = ogjk.DeviceScene(b)
+    sc.build_clusters(512)
+    slot = sc.cslot.cpu().numpy()
+    assert ((slot == -1) == (b.cnt < 512)).all()
+    for cmin in (512, 1):
+        sc.set_cluster_min(cmin)
+        out = sc.collide(oracle.GJK_EPA, b.n_pairs, f64_record=True)
+        torch.cuda.synchronize()
+        res = compare_f64(out["f64"].cpu().numpy(), ogjk.simplices_from_device(out["simplices"]),
+                          ref, oracle.GJK_EPA)
+        check(res, f"clusters from 512, threshold {cmin}")
 
 
 # Size-bucketed launch tables (device partition, per-bucket lanes, EPA lane
