@@ -90,16 +90,22 @@ __global__ void __launch_bounds__(256) k_set_super(const int64_t* __restrict__ b
   if (q < n) body_super[bs[2 * q]] = bs[2 * q + 1];
 }
 
-// Sort keys: Hilbert index on a 2^kGridBits x 2^kGridBits grid over the
-// octahedral map of directions (4096^2 cells: far finer than a cluster of 32
-// vertices of the largest bodies), kKeyBits bits for the radix sorts.
-constexpr int kGridBits = 12;
-constexpr int kKeyBits = 2 * kGridBits;
+// Sort keys: Hilbert index on a 2^G x 2^G grid over the octahedral map of
+// directions.  Bodies of at most kLdsBodyMax vertices take G = 8 (256^2
+// cells, about 500 per cluster of a 4,096-vertex body, and two 8-bit passes
+// of the block sort), larger ones G = 12 (4096^2 cells, finer than a cluster
+// of bodies of 10^5 vertices; their device sort runs over kKeyBits bits).
+// The grid depends only on the body's size, so both build paths key a body
+// alike; vertices sharing a cell keep their input order.
+constexpr int kGridBitsSmall = 8, kGridBitsLarge = 12;
+constexpr int kKeyBits = 2 * kGridBitsLarge;
 
-// Index of (x, y) on the Hilbert curve over the grid.
+// Index of (x, y) on the Hilbert curve over the 2^G x 2^G grid.
+template <int G>
 OGJK_DEV uint32_t hilbert_key(uint32_t x, uint32_t y) {
-  constexpr uint32_t n = 1u << kGridBits;
+  constexpr uint32_t n = 1u << G;
   uint32_t d = 0;
+#pragma unroll
   for (uint32_t s = n / 2; s > 0; s >>= 1) {
     const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
     d += s * s * ((3u * rx) ^ ry);
@@ -116,8 +122,10 @@ OGJK_DEV uint32_t hilbert_key(uint32_t x, uint32_t y) {
   return d;
 }
 
-// Octahedral map of the direction v (any length) to the grid's cells.
-OGJK_DEV void octa_cell(float vx, float vy, float vz, uint32_t& qx, uint32_t& qy) {
+// Key of the direction v (any length): its cell on the octahedral map of
+// the 2^G x 2^G grid, then that cell's Hilbert index.
+template <int G>
+OGJK_DEV uint32_t direction_key(float vx, float vy, float vz) {
   const float l1 = fabsf(vx) + fabsf(vy) + fabsf(vz);
   float u = 0.0f, w = 0.0f;
   if (l1 > 0.0f) {
@@ -130,10 +138,11 @@ OGJK_DEV void octa_cell(float vx, float vy, float vz, uint32_t& qx, uint32_t& qy
       w = ww;
     }
   }
-  constexpr float half = 0.5f * (float)(1u << kGridBits) - 0.5f, top = (float)((1u << kGridBits) - 1);
+  constexpr float half = 0.5f * (float)(1u << G) - 0.5f, top = (float)((1u << G) - 1);
   const float fu = (u + 1.0f) * half, fw = (w + 1.0f) * half;
-  qx = fu > 0.0f ? (uint32_t)fminf(fu, top) : 0u;  // NaN -> 0
-  qy = fw > 0.0f ? (uint32_t)fminf(fw, top) : 0u;
+  const uint32_t qx = fu > 0.0f ? (uint32_t)fminf(fu, top) : 0u;  // NaN -> 0
+  const uint32_t qy = fw > 0.0f ? (uint32_t)fminf(fw, top) : 0u;
+  return hilbert_key<G>(qx, qy);
 }
 
 OGJK_DEV float block_min(float v, float* red) {
@@ -181,10 +190,11 @@ __global__ void __launch_bounds__(256) k_order_keys(const float* __restrict__ x,
     c[k] = 0.5f * lo[k] + 0.5f * hi[k];
   }
   const int base = kbeg[blockIdx.x];
+  const bool small = n <= kLdsBodyMax;  // uniform over the block
   for (int i = threadIdx.x; i < n; i += 256) {
-    uint32_t qx, qy;
-    octa_cell(x[o + i] - c[0], y[o + i] - c[1], z[o + i] - c[2], qx, qy);
-    keys[base + i] = hilbert_key(qx, qy);
+    const float vx = x[o + i] - c[0], vy = y[o + i] - c[1], vz = z[o + i] - c[2];
+    keys[base + i] = small ? direction_key<kGridBitsSmall>(vx, vy, vz)
+                           : direction_key<kGridBitsLarge>(vx, vy, vz);
     vals[base + i] = i;
   }
 }
@@ -244,11 +254,33 @@ __global__ void __launch_bounds__(256) k_cluster_pad(const int32_t* __restrict__
   }
 }
 
+// The value of lane CTRL-permuted within each 16-lane row (DPP: CTRL 0xB1
+// pairs lanes i and i ^ 1, 0x4E lanes i and i ^ 2, 0x141 mirrors each 8-lane
+// half, 0x140 each row).
+template <int CTRL>
+OGJK_DEV float row_perm(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// op over the 32-lane slice (kCluster lanes), in every lane of it, with the
+// whole wave active: four DPP steps inside each 16-lane row, then
+// v_permlane16_swap exchanges the slice's two rows.  No LDS traffic.
+template <class Op>
+OGJK_DEV float slice_reduce(float v, Op op) {
+  static_assert(kCluster == 32, "a slice is two 16-lane rows");
+  v = op(v, row_perm<0xB1>(v));
+  v = op(v, row_perm<0x4E>(v));
+  v = op(v, row_perm<0x141>(v));
+  v = op(v, row_perm<0x140>(v));
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+  return op(__uint_as_float(r[0]), __uint_as_float(r[1]));
+}
+
 // Bounds of one cluster held by a kCluster-lane slice, lane t holding slot t
-// (valid: a vertex, not padding): centre c = midpoint of the bounding box
-// (rounded to float), radius r = max |x - c| and half extents
-// h_k = max |x_k - c_k|, computed in FP64 and rounded up to float.  Every
-// lane of the slice returns the bound.
+// (valid: a vertex, not padding), with the whole wave active: centre c =
+// midpoint of the bounding box (rounded to float), radius r = max |x - c| and
+// half extents h_k = max |x_k - c_k|, computed in FP64 and rounded up to
+// float.  Every lane of the slice returns the bound.
 OGJK_DEV ClusterBound slice_bound(bool valid, const float (&p)[3]) {
   float lo[3], hi[3];
 #pragma unroll
@@ -256,13 +288,12 @@ OGJK_DEV ClusterBound slice_bound(bool valid, const float (&p)[3]) {
     lo[k] = valid ? p[k] : INFINITY;
     hi[k] = valid ? p[k] : -INFINITY;
   }
+  const auto fmin_ = [](float a, float b) { return fminf(a, b); };
+  const auto fmax_ = [](float a, float b) { return fmaxf(a, b); };
 #pragma unroll
-  for (int m = 1; m < kCluster; m <<= 1) {
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      lo[k] = fminf(lo[k], __shfl_xor(lo[k], m, kCluster));
-      hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], m, kCluster));
-    }
+  for (int k = 0; k < 3; k++) {
+    lo[k] = slice_reduce(lo[k], fmin_);
+    hi[k] = slice_reduce(hi[k], fmax_);
   }
   float ctr[3];
   double e[3], r2 = 0.0;
@@ -272,19 +303,19 @@ OGJK_DEV ClusterBound slice_bound(bool valid, const float (&p)[3]) {
     e[k] = valid ? fabs((double)p[k] - (double)ctr[k]) : 0.0;
   }
   if (valid) r2 = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+  // Each lane rounds its own distances up to float, then the slice takes the
+  // maximum: the correctly rounded square root, the conversion to float and
+  // the step up are all monotone, so this is bit for bit the rounded-up
+  // maximum, with 32-bit rather than 64-bit exchanges.
+  float rr = nextafterf((float)sqrt(r2), INFINITY), hh[3];
 #pragma unroll
-  for (int m = 1; m < kCluster; m <<= 1) {
-    r2 = fmax(r2, __shfl_xor(r2, m, kCluster));
+  for (int k = 0; k < 3; k++) hh[k] = nextafterf((float)e[k], INFINITY);
+  rr = slice_reduce(rr, fmax_);
 #pragma unroll
-    for (int k = 0; k < 3; k++) e[k] = fmax(e[k], __shfl_xor(e[k], m, kCluster));
-  }
-  // a correctly rounded square root is monotone: the root of the largest
-  // squared distance is the largest root, bit for bit
-  const double r = sqrt(r2);
+  for (int k = 0; k < 3; k++) hh[k] = slice_reduce(hh[k], fmax_);
   ClusterBound cb;
-  cb.c = make_float4(ctr[0], ctr[1], ctr[2], nextafterf((float)r, INFINITY));
-  cb.h = make_float4(nextafterf((float)e[0], INFINITY), nextafterf((float)e[1], INFINITY),
-                     nextafterf((float)e[2], INFINITY), 0.0f);
+  cb.c = make_float4(ctr[0], ctr[1], ctr[2], rr);
+  cb.h = make_float4(hh[0], hh[1], hh[2], 0.0f);
   return cb;
 }
 
@@ -358,11 +389,8 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_body(
   for (int j = 0; j < IPT; j++) {
     const int i = j * BLOCK + t;
     key[j] = 0xffffffffu;  // padding sorts after every vertex (stable: after equal keys)
-    if (i < n) {
-      uint32_t qx, qy;
-      octa_cell(x[o + i] - c[0], y[o + i] - c[1], z[o + i] - c[2], qx, qy);
-      key[j] = hilbert_key(qx, qy);
-    }
+    if (i < n)
+      key[j] = direction_key<kGridBitsSmall>(x[o + i] - c[0], y[o + i] - c[1], z[o + i] - c[2]);
   }
   // blocked order (thread t: items t*IPT ...) is index order, which a stable
   // sort keeps among equal keys, as the segmented sort does
@@ -372,7 +400,7 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_body(
   }
 #pragma unroll
   for (int j = 0; j < IPT; j++) val[j] = t * IPT + j;
-  Sort(tmp.sort).SortBlockedToStriped(key, val, 0, kKeyBits);
+  Sort(tmp.sort).SortBlockedToStriped(key, val, 0, 2 * kGridBitsSmall);
 
   // rank j*BLOCK + t: the copy (coalesced) and, per 32-lane slice, one cluster
   const int padded = (n + kCluster - 1) / kCluster * kCluster;
