@@ -123,14 +123,17 @@ OGJK_DEV uint32_t hilbert_key(uint32_t x, uint32_t y) {
 }
 
 // Key of the direction v (any length): its cell on the octahedral map of
-// the 2^G x 2^G grid, then that cell's Hilbert index.
+// the 2^G x 2^G grid, then that cell's Hilbert index.  (An 8-bit key, one
+// radix pass, built C2's clusters 2.5 ms faster and its queries 1 ms slower.)
 template <int G>
 OGJK_DEV uint32_t direction_key(float vx, float vy, float vz) {
   const float l1 = fabsf(vx) + fabsf(vy) + fabsf(vz);
   float u = 0.0f, w = 0.0f;
   if (l1 > 0.0f) {
-    u = vx / l1;
-    w = vy / l1;
+    // the key only groups nearby directions: the hardware reciprocal is enough
+    const float inv = __builtin_amdgcn_rcpf(l1);
+    u = vx * inv;
+    w = vy * inv;
     if (vz < 0.0f) {
       const float uu = (1.0f - fabsf(w)) * (u >= 0.0f ? 1.0f : -1.0f);
       const float ww = (1.0f - fabsf(u)) * (w >= 0.0f ? 1.0f : -1.0f);
