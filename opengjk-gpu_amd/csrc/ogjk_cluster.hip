@@ -18,12 +18,20 @@ namespace {
 
 // Bodies of at most this many vertices are built by one block each, sorted in
 // LDS (at most 256 threads x 16 keys), in size classes of at most 64, 256,
-// 1024, 2048 and 4096 vertices.
+// 768, 1024, 1536, 2048, 3072 and 4096 vertices (a block sorts its class's
+// maximum, padding included).
 constexpr int kLdsBodyMax = 4096;
-constexpr int kClasses = 5;
+constexpr int kClasses = 8;
 
 OGJK_DEV int body_class(int n) {
-  return n <= 64 ? 0 : (n <= 256 ? 1 : (n <= 1024 ? 2 : (n <= 2048 ? 3 : 4)));
+  return n <= 64     ? 0
+         : n <= 256  ? 1
+         : n <= 768  ? 2
+         : n <= 1024 ? 3
+         : n <= 1536 ? 4
+         : n <= 2048 ? 5
+         : n <= 3072 ? 6
+                     : 7;
 }
 
 // Per body: its padded slot count when it is built (n >= min_build), else 0,
@@ -544,7 +552,7 @@ hipError_t build_clusters(const float* x, const float* y, const float* z, const 
   if (n_bodies == 0x7fffffff) return hipErrorInvalidValue;  // n_bodies + 1 scan entries
   min_build = std::max(min_build, 1);
   // Bodies of at most lds_max vertices are built whole by one block each
-  // (k_cluster_body, five size classes); larger ones go through batches of
+  // (k_cluster_body, eight size classes); larger ones go through batches of
   // keys, a device segmented sort and a scatter.  OGJK_CLUSTER_LDS_MAX=0 sends
   // every body through the batches (the two paths give identical arrays).
   int lds_max = kLdsBodyMax;
@@ -617,9 +625,12 @@ hipError_t build_clusters(const float* x, const float* y, const float* z, const 
     };
     launch(k_cluster_body<64, 1>, 0, 64);
     launch(k_cluster_body<64, 4>, 1, 64);
-    launch(k_cluster_body<256, 4>, 2, 256);
-    launch(k_cluster_body<256, 8>, 3, 256);  // 512-thread blocks measured slower
-    launch(k_cluster_body<256, 16>, 4, 256);
+    launch(k_cluster_body<256, 3>, 2, 256);
+    launch(k_cluster_body<256, 4>, 3, 256);
+    launch(k_cluster_body<256, 6>, 4, 256);
+    launch(k_cluster_body<256, 8>, 5, 256);  // 512-thread blocks measured slower
+    launch(k_cluster_body<256, 12>, 6, 256);
+    launch(k_cluster_body<256, 16>, 7, 256);
     if ((e = hipGetLastError()) != hipSuccess) return fail_with(e);
   }
 

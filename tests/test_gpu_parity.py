@@ -160,12 +160,12 @@ def test_cluster_build_paths_identical(torch_cuda):
     """Bodies of at most 4,096 vertices are built by one block each (keys, a
     stable sort in LDS, copy and bounds in one kernel); the batched path (keys,
     device segmented sort, scatter, bounds kernel) must give the same arrays
-    bit for bit.  Sizes straddle the block classes (1024, 2048, 4096) and the
+    bit for bit.  Sizes straddle the block classes (256 ... 4096) and the
     batched bodies; repeated points give equal keys, so both sorts must keep
     index order among them."""
     rng = np.random.default_rng(12)
-    sizes = [1, 2, 3, 31, 32, 33, 255, 256, 257, 1023, 1024, 1025, 2047, 2048, 2049, 4095, 4096,
-             4097, 5000, 9000]
+    sizes = [1, 2, 3, 31, 32, 33, 255, 256, 257, 767, 768, 769, 1023, 1024, 1025, 1535, 1536, 1537,
+             2047, 2048, 2049, 3071, 3072, 3073, 4095, 4096, 4097, 5000, 9000]
     A, B = [], []
     for n in sizes:
         for side in (A, B):
