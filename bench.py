@@ -49,7 +49,9 @@ PIPELINE, FUSED, SPLIT = 0, 1, 2  # launch shapes (ogjk_set_config)
 # 3 GJK then EPA; lanes: (GJK, EPA) lanes per pair; cluster_min: bodies with at
 # least this many vertices are swept through their clusters (None: no
 # clusters); profile: the rocprofv3 summary holding the measured HBM traffic
-# of one step; graph: also time the step replayed as one HIP graph.
+# of one step; graph: also time the step replayed as one HIP graph; min_steps:
+# at least this many timed steps, mixed and plain alike (a 30 us step timed
+# 10 times is mostly launch jitter).
 WORKLOADS = {
     "ellipsoid_uneven": dict(
         cfg="C2", mode=3, gen=dict(family=2, n_pairs=1_000_000, seed=2, box=1.5, nmin=4, nmax=4096),
@@ -66,7 +68,8 @@ WORKLOADS = {
         cfg="C0", mode=1, gen=dict(family=0, n_pairs=1000, seed=0, nverts=32, box=2.0),
         desc="1000 pairs x 32-vertex sphere polytopes, radius U[0.5,1.5], centres U[-2,2]^3 "
              "(the literal spec: 25% overlap), GJK distance only",
-        lanes=(64, 32), cluster_min=None, profile="gjk_sphere32_1k.md", graph=False),
+        lanes=(64, 32), cluster_min=None, profile="gjk_sphere32_1k.md", graph=False,
+        min_steps=200),
     "noisy_sphere_large": dict(
         cfg="C3", mode=3,
         gen=dict(family=3, n_pairs=10_000, seed=3, nverts=100_000, box=1.5, sigma=0.01),
@@ -255,6 +258,7 @@ def batch_leg(name: str, args, headline: bool) -> dict:
     spec = WORKLOADS[name]
     mode = spec["mode"]
     steps = args.steps if headline else max(2, min(args.steps, 10))
+    steps = max(steps, spec.get("min_steps", 0))
     warmup = args.warmup if headline else min(args.warmup, 2)
     lg, le = (int(x) for x in args.lanes.split(",")) if (args.lanes and headline) else spec["lanes"]
     cmin = spec["cluster_min"]

@@ -927,7 +927,9 @@ static hipError_t launch_bucketed(int mode, const Src& src, const OutArgs& o, in
   const LaunchConfig cfg = g_cfg;
   const BucketPlan p = plan_of(cfg);
   unsigned* c = (unsigned*)ws;
-  hipLaunchKernelGGL(k_zero, dim3(1), dim3(256), 0, st, c);
+  // GJK alone over a single bucket reads and writes no counter (no
+  // partition, no EPA jobs): then there is nothing to clear
+  if (mode == 3 || p.nb > 1) hipLaunchKernelGGL(k_zero, dim3(1), dim3(256), 0, st, c);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (p.nb > 1) {
