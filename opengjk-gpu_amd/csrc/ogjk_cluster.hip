@@ -226,16 +226,30 @@ __global__ void __launch_bounds__(256) k_cluster_scatter(
   // the grid is a multiple of 8 blocks, so this is a bijection of [0, grid)
   const unsigned per = gridDim.x / 8u;
   const unsigned lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
-  const int64_t p = (int64_t)lb * 256 + threadIdx.x;
+  const int64_t p0 = (int64_t)lb * 256;
+  // last q with kbeg[q] <= pos, searched in [lo, nb - 1] (kbeg[lo] <= pos)
+  auto last_start = [&](int lo, int64_t pos) {
+    int hi = nb - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (kbeg[mid] <= pos) lo = mid;
+      else hi = mid - 1;
+    }
+    return lo;
+  };
+  // one search for the block's first position; a block rarely spans more
+  // than one body, so each thread steps on from there (a few steps, then a
+  // search when the block crosses many small bodies)
+  __shared__ int q_first;
+  if (threadIdx.x == 0) q_first = last_start(0, p0);
+  __syncthreads();
+  const int64_t p = p0 + threadIdx.x;
   if (p >= nv) return;
-  int lo = 0, hi = nb - 1;  // last q with kbeg[q] <= p
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (kbeg[mid] <= p) lo = mid;
-    else hi = mid - 1;
-  }
-  const int b = blist[b0 + lo];
-  const int i = (int)(p - kbeg[lo]);
+  int q = q_first;
+  for (int k = 0; k < 4 && q + 1 < nb && kbeg[q + 1] <= p; k++) q++;
+  if (q + 1 < nb && kbeg[q + 1] <= p) q = last_start(q + 1, p);
+  const int b = blist[b0 + q];
+  const int i = (int)(p - kbeg[q]);
   const int l = sorted[p];
   const int64_t o = off[b], s = slot[b] + i;
   cx[s] = x[o + l];
