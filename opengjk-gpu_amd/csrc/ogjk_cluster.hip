@@ -465,30 +465,35 @@ __global__ void __launch_bounds__(256) k_cluster_bounds(const float* __restrict_
   if (in && t == 0) bounds[c] = cb;
 }
 
-OGJK_DEV double block_max_d(double v, double* red) {
-  for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  return fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+// op over the whole wave, in every lane: the slice reduction, then
+// v_permlane32_swap exchanges the wave's two halves.
+template <class Op>
+OGJK_DEV float wave_reduce(float v, Op op) {
+  v = slice_reduce(v, op);
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+  return op(__uint_as_float(r[0]), __uint_as_float(r[1]));
 }
 
-// One block per supercluster (first slot, slot count <= kSuper*kCluster):
-// sphere and box bounds of its vertices, as k_cluster_bounds does for one
-// cluster.
+// One wave per supercluster (first slot, slot count <= kSuper*kCluster), four
+// per block: sphere and box bounds of its vertices, made as a cluster's are
+// (each lane rounds its own FP64 distances up to float before the maximum).
 __global__ void __launch_bounds__(256) k_super_bounds(const float* __restrict__ cx,
                                                       const float* __restrict__ cy,
                                                       const float* __restrict__ cz,
                                                       const int32_t* __restrict__ orig,
                                                       const int64_t* __restrict__ first,
                                                       const int32_t* __restrict__ count,
+                                                      int64_t n_supers,
                                                       ClusterBound* __restrict__ sbounds) {
-  __shared__ float red[4];
-  __shared__ double redd[4];
-  const int64_t s0 = first[blockIdx.x];
-  const int n = count[blockIdx.x];
+  const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (q >= n_supers) return;  // the whole wave: the reductions below need every lane
+  const int lane = threadIdx.x & 63;
+  const int64_t s0 = first[q];
+  const int n = count[q];
+  const auto fmin_ = [](float a, float b) { return fminf(a, b); };
+  const auto fmax_ = [](float a, float b) { return fmaxf(a, b); };
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int i = threadIdx.x; i < n; i += 256) {
+  for (int i = lane; i < n; i += 64) {
     if (orig[s0 + i] < 0) continue;
     const float p[3] = {cx[s0 + i], cy[s0 + i], cz[s0 + i]};
 #pragma unroll
@@ -500,12 +505,12 @@ __global__ void __launch_bounds__(256) k_super_bounds(const float* __restrict__ 
   float ctr[3];
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    lo[k] = block_min(lo[k], red);
-    hi[k] = block_max(hi[k], red);
+    lo[k] = wave_reduce(lo[k], fmin_);
+    hi[k] = wave_reduce(hi[k], fmax_);
     ctr[k] = 0.5f * lo[k] + 0.5f * hi[k];
   }
   double r2 = 0.0, e[3] = {0.0, 0.0, 0.0};
-  for (int i = threadIdx.x; i < n; i += 256) {
+  for (int i = lane; i < n; i += 64) {
     if (orig[s0 + i] < 0) continue;
     const double ex = fabs((double)cx[s0 + i] - (double)ctr[0]);
     const double ey = fabs((double)cy[s0 + i] - (double)ctr[1]);
@@ -515,15 +520,15 @@ __global__ void __launch_bounds__(256) k_super_bounds(const float* __restrict__ 
     e[1] = fmax(e[1], ey);
     e[2] = fmax(e[2], ez);
   }
-  const double r = sqrt(block_max_d(r2, redd));  // as in slice_bound
+  const float rr = wave_reduce(nextafterf((float)sqrt(r2), INFINITY), fmax_);
+  float hh[3];
 #pragma unroll
-  for (int k = 0; k < 3; k++) e[k] = block_max_d(e[k], redd);
-  if (threadIdx.x == 0) {
+  for (int k = 0; k < 3; k++) hh[k] = wave_reduce(nextafterf((float)e[k], INFINITY), fmax_);
+  if (lane == 0) {
     ClusterBound cb;
-    cb.c = make_float4(ctr[0], ctr[1], ctr[2], nextafterf((float)r, INFINITY));
-    cb.h = make_float4(nextafterf((float)e[0], INFINITY), nextafterf((float)e[1], INFINITY),
-                       nextafterf((float)e[2], INFINITY), 0.0f);
-    sbounds[blockIdx.x] = cb;
+    cb.c = make_float4(ctr[0], ctr[1], ctr[2], rr);
+    cb.h = make_float4(hh[0], hh[1], hh[2], 0.0f);
+    sbounds[q] = cb;
   }
 }
 
@@ -804,8 +809,8 @@ hipError_t build_clusters(const float* x, const float* y, const float* z, const 
             hipSuccess &&
         (e = hipMemcpyAsync(d_count, scount.data(), 4 * ns, hipMemcpyHostToDevice, st)) ==
             hipSuccess) {
-      hipLaunchKernelGGL(k_super_bounds, dim3((unsigned)ns), dim3(256), 0, st, cx, cy, cz, orig,
-                         d_first, d_count, (ClusterBound*)sbounds);
+      hipLaunchKernelGGL(k_super_bounds, dim3((unsigned)((ns + 3) / 4)), dim3(256), 0, st, cx, cy,
+                         cz, orig, d_first, d_count, (int64_t)ns, (ClusterBound*)sbounds);
       e = hipGetLastError();
       if (e == hipSuccess) e = hipStreamSynchronize(st);
     }
