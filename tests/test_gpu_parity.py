@@ -362,6 +362,31 @@ def test_host_staging_reuse(torch_cuda):
     assert (d < 0).any() and (d > 0).any()
 
 
+def test_host_staging_concurrent_callers(torch_cuda):
+    """Threads calling the host entry point at once share one device's kept
+    staging (the call holds it until its results are copied out): every
+    thread's results equal the same call made alone."""
+    import threading
+    batches = [gen.make(gen.SPHERE, 200 + 700 * k, seed=70 + k, nverts=24 + 16 * k, box=1.4)
+               for k in range(4)]
+    alone = [ogjk.collide_host(oracle.GJK_EPA, b) for b in batches]
+    got = [[None] * 3 for _ in batches]
+
+    def run(k):
+        for r in range(3):
+            got[k][r] = ogjk.collide_host(oracle.GJK_EPA, batches[k])
+
+    th = [threading.Thread(target=run, args=(k,)) for k in range(len(batches))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for k, ref in enumerate(alone):
+        for out in got[k]:
+            for a, b in zip(out, ref):
+                assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8)), k
+
+
 def test_invalid_pairs_flagged(torch_cuda):
     b = gen.make(gen.SPHERE, 8, seed=9, nverts=16, box=1.0)
     pairs = np.array([[0, 1], [2, 99], [-1, 3], [4, 5]], np.int32)
