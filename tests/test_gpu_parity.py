@@ -326,6 +326,25 @@ def test_host_paths_agree(torch_cuda):
     assert np.array_equal(ni.view(np.uint32), n1.view(np.uint32))
 
 
+def test_cluster_threshold_above_every_body(torch_cuda):
+    """A build threshold no body reaches builds nothing (no slots, no
+    superclusters, every body_slot -1); queries sweep every body whole and
+    stay exact."""
+    torch = torch_cuda
+    b = gen.make(gen.ELLIPSOID, 300, seed=14, box=1.5, nmin=4, nmax=6000)
+    ref = oracle_run(oracle.GJK_EPA, b)
+    sc = ogjk.DeviceScene(b)
+    sc.build_clusters(10**6)
+    torch.cuda.synchronize()
+    assert (sc.cslot.cpu().numpy() == -1).all() and (sc.csuper.cpu().numpy() == -1).all()
+    assert sc.clusters.n_supers_used == 0
+    out = sc.collide(oracle.GJK_EPA, b.n_pairs, f64_record=True)
+    torch.cuda.synchronize()
+    res = compare_f64(out["f64"].cpu().numpy(), ogjk.simplices_from_device(out["simplices"]), ref,
+                      oracle.GJK_EPA)
+    check(res, "clusters from 10^6 vertices (none built)")
+
+
 def test_host_staging_reuse(torch_cuda):
     """The host entry points keep their staging buffers between calls: calls
     that grow, shrink and grow again, in every mode, each against the oracle,
