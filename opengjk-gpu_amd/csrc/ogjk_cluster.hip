@@ -107,6 +107,8 @@ __global__ void __launch_bounds__(256) k_set_super(const int64_t* __restrict__ b
 // alike; vertices sharing a cell keep their input order.
 constexpr int kGridBitsSmall = 8, kGridBitsLarge = 12;
 constexpr int kKeyBits = 2 * kGridBitsLarge;
+// The one-block sort packs a body-local index (< kLdsBodyMax) below the key.
+constexpr int kIndexBits = 12;
 
 // Index of (x, y) on the Hilbert curve over the 2^G x 2^G grid.
 template <int G>
@@ -376,7 +378,8 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_body(
     float* __restrict__ cy, float* __restrict__ cz, int32_t* __restrict__ orig,
     ClusterBound* __restrict__ bounds) {
   using Exchange = hipcub::BlockExchange<uint32_t, BLOCK, IPT>;
-  using Sort = hipcub::BlockRadixSort<uint32_t, BLOCK, IPT, int>;
+  using Sort = hipcub::BlockRadixSort<uint32_t, BLOCK, IPT>;  // keys only: the index rides in them
+  static_assert(kLdsBodyMax <= (1 << kIndexBits), "a body-local index fits below the key");
   __shared__ union {
     typename Exchange::TempStorage ex;
     typename Sort::TempStorage sort;
@@ -408,14 +411,17 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_body(
     hi[k] = -group_min_blk<BLOCK>(-hi[k], red);
     c[k] = 0.5f * lo[k] + 0.5f * hi[k];
   }
+  // item (key << kIndexBits) | index, sorted on the key bits only: the index
+  // travels with its key and no value array goes through the sort
   uint32_t key[IPT];
-  int val[IPT];
 #pragma unroll
   for (int j = 0; j < IPT; j++) {
     const int i = j * BLOCK + t;
     key[j] = 0xffffffffu;  // padding sorts after every vertex (stable: after equal keys)
     if (i < n)
-      key[j] = direction_key<kGridBitsSmall>(x[o + i] - c[0], y[o + i] - c[1], z[o + i] - c[2]);
+      key[j] = direction_key<kGridBitsSmall>(x[o + i] - c[0], y[o + i] - c[1], z[o + i] - c[2])
+                   << kIndexBits |
+               (uint32_t)i;
   }
   // blocked order (thread t: items t*IPT ...) is index order, which a stable
   // sort keeps among equal keys, as the segmented sort does
@@ -423,9 +429,7 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_body(
     Exchange(tmp.ex).StripedToBlocked(key);
     __syncthreads();
   }
-#pragma unroll
-  for (int j = 0; j < IPT; j++) val[j] = t * IPT + j;
-  Sort(tmp.sort).SortBlockedToStriped(key, val, 0, 2 * kGridBitsSmall);
+  Sort(tmp.sort).SortBlockedToStriped(key, kIndexBits, kIndexBits + 2 * kGridBitsSmall);
 
   // rank j*BLOCK + t: the copy (coalesced) and, per 32-lane slice, one cluster
   const int padded = (n + kCluster - 1) / kCluster * kCluster;
@@ -434,7 +438,7 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_body(
     const int r = j * BLOCK + t;
     if (j * BLOCK >= padded) break;  // uniform over the block
     const bool valid = r < n;
-    const int l = val[j];
+    const int l = (int)(key[j] & ((1u << kIndexBits) - 1u));
     const float p[3] = {valid ? x[o + l] : 0.0f, valid ? y[o + l] : 0.0f,
                         valid ? z[o + l] : 0.0f};
     if (r < padded) {
