@@ -303,6 +303,15 @@ OGJK_DEV float slice_reduce(float v, Op op) {
   return op(__uint_as_float(r[0]), __uint_as_float(r[1]));
 }
 
+// op over the whole wave, in every lane: the slice reduction, then
+// v_permlane32_swap exchanges the wave's two halves.
+template <class Op>
+OGJK_DEV float wave_reduce(float v, Op op) {
+  v = slice_reduce(v, op);
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+  return op(__uint_as_float(r[0]), __uint_as_float(r[1]));
+}
+
 // Bounds of one cluster held by a kCluster-lane slice, lane t holding slot t
 // (valid: a vertex, not padding), with the whole wave active: centre c =
 // midpoint of the bounding box (rounded to float), radius r = max |x - c| and
@@ -346,21 +355,40 @@ OGJK_DEV ClusterBound slice_bound(bool valid, const float (&p)[3]) {
   return cb;
 }
 
-// Minimum over a block of BLOCK threads (a multiple of the wave); red holds
-// BLOCK / 64 floats.
+// Minimum of lo[k] and maximum of hi[k] (k = 0..2) over a block of BLOCK
+// threads (a multiple of the wave), in every thread, with the whole block
+// active: each wave reduces with DPP and lane permutes, then the waves' six
+// values pass once through red (6 * BLOCK / 64 floats) behind one barrier.
 template <int BLOCK>
-OGJK_DEV float group_min_blk(float v, float* red) {
-  for (int m = 32; m >= 1; m >>= 1) v = fminf(v, __shfl_xor(v, m, 64));
-  if constexpr (BLOCK == 64) {
-    return v;
-  } else {
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    float r = red[0];
+OGJK_DEV void block_box(float (&lo)[3], float (&hi)[3], float* red) {
+  const auto fmin_ = [](float a, float b) { return fminf(a, b); };
+  const auto fmax_ = [](float a, float b) { return fmaxf(a, b); };
 #pragma unroll
-    for (int w = 1; w < BLOCK / 64; w++) r = fminf(r, red[w]);
-    return r;
+  for (int k = 0; k < 3; k++) {
+    lo[k] = wave_reduce(lo[k], fmin_);
+    hi[k] = wave_reduce(hi[k], fmax_);
+  }
+  if constexpr (BLOCK > 64) {
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        red[6 * w + k] = lo[k];
+        red[6 * w + 3 + k] = hi[k];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      float a = red[k], b = red[3 + k];
+#pragma unroll
+      for (int u = 1; u < BLOCK / 64; u++) {
+        a = fminf(a, red[6 * u + k]);
+        b = fmaxf(b, red[6 * u + 3 + k]);
+      }
+      lo[k] = a;
+      hi[k] = b;
+    }
   }
 }
 
@@ -384,7 +412,7 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_body(
     typename Exchange::TempStorage ex;
     typename Sort::TempStorage sort;
   } tmp;
-  __shared__ float red[BLOCK / 64];
+  __shared__ float red[6 * (BLOCK / 64)];
   const int b = blist[blockIdx.x];
   const int n = cnt[b];  // 0 < n <= BLOCK * IPT: the host's classes
   const int64_t o = off[b], s = slot[b];
@@ -405,12 +433,9 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_body(
     }
   }
   float c[3];
+  block_box<BLOCK>(lo, hi, red);
 #pragma unroll
-  for (int k = 0; k < 3; k++) {
-    lo[k] = group_min_blk<BLOCK>(lo[k], red);
-    hi[k] = -group_min_blk<BLOCK>(-hi[k], red);
-    c[k] = 0.5f * lo[k] + 0.5f * hi[k];
-  }
+  for (int k = 0; k < 3; k++) c[k] = 0.5f * lo[k] + 0.5f * hi[k];
   // item (key << kIndexBits) | index, sorted on the key bits only: the index
   // travels with its key and no value array goes through the sort
   uint32_t key[IPT];
@@ -469,14 +494,6 @@ __global__ void __launch_bounds__(256) k_cluster_bounds(const float* __restrict_
   if (in && t == 0) bounds[c] = cb;
 }
 
-// op over the whole wave, in every lane: the slice reduction, then
-// v_permlane32_swap exchanges the wave's two halves.
-template <class Op>
-OGJK_DEV float wave_reduce(float v, Op op) {
-  v = slice_reduce(v, op);
-  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
-  return op(__uint_as_float(r[0]), __uint_as_float(r[1]));
-}
 
 // One wave per supercluster (first slot, slot count <= kSuper*kCluster), four
 // per block: sphere and box bounds of its vertices, made as a cluster's are
