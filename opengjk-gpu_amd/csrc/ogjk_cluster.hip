@@ -418,13 +418,19 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_body(
   const int64_t o = off[b], s = slot[b];
   const int t = threadIdx.x;
 
-  // bounding-box centre, then the keys (item j of thread t: vertex j*BLOCK + t)
+  // bounding-box centre, then the keys (item j of thread t: vertex j*BLOCK + t),
+  // the coordinates kept in registers between the two (measured faster than
+  // loading them again, even at 16 items a thread)
+  float px[IPT], py[IPT], pz[IPT];
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
 #pragma unroll
   for (int j = 0; j < IPT; j++) {
     const int i = j * BLOCK + t;
     if (i < n) {
       const float p[3] = {x[o + i], y[o + i], z[o + i]};
+      px[j] = p[0];
+      py[j] = p[1];
+      pz[j] = p[2];
 #pragma unroll
       for (int k = 0; k < 3; k++) {
         lo[k] = fminf(lo[k], p[k]);
@@ -443,10 +449,11 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_body(
   for (int j = 0; j < IPT; j++) {
     const int i = j * BLOCK + t;
     key[j] = 0xffffffffu;  // padding sorts after every vertex (stable: after equal keys)
-    if (i < n)
-      key[j] = direction_key<kGridBitsSmall>(x[o + i] - c[0], y[o + i] - c[1], z[o + i] - c[2])
+    if (i < n) {
+      key[j] = direction_key<kGridBitsSmall>(px[j] - c[0], py[j] - c[1], pz[j] - c[2])
                    << kIndexBits |
                (uint32_t)i;
+    }
   }
   // blocked order (thread t: items t*IPT ...) is index order, which a stable
   // sort keeps among equal keys, as the segmented sort does
