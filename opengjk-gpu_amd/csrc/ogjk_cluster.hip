@@ -405,27 +405,27 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_body(
     const int64_t* __restrict__ slot, const int* __restrict__ blist, float* __restrict__ cx,
     float* __restrict__ cy, float* __restrict__ cz, int32_t* __restrict__ orig,
     ClusterBound* __restrict__ bounds) {
-  using Exchange = hipcub::BlockExchange<uint32_t, BLOCK, IPT>;
-  using Sort = hipcub::BlockRadixSort<uint32_t, BLOCK, IPT>;  // keys only: the index rides in them
+  // Keys only (the index rides below the key), ranked a wave at a time by
+  // matching; the input is warp-striped -- item j of lane ln in wave wv is
+  // element (wv * IPT + j) * 64 + ln -- which is index order, so no exchange
+  // precedes the sort.
+  using Sort = ::rocprim::block_radix_sort<uint32_t, BLOCK, IPT>;
   static_assert(kLdsBodyMax <= (1 << kIndexBits), "a body-local index fits below the key");
-  __shared__ union {
-    typename Exchange::TempStorage ex;
-    typename Sort::TempStorage sort;
-  } tmp;
+  __shared__ typename Sort::storage_type sort_storage;
   __shared__ float red[6 * (BLOCK / 64)];
   const int b = blist[blockIdx.x];
   const int n = cnt[b];  // 0 < n <= BLOCK * IPT: the host's classes
   const int64_t o = off[b], s = slot[b];
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
 
-  // bounding-box centre, then the keys (item j of thread t: vertex j*BLOCK + t),
-  // the coordinates kept in registers between the two (measured faster than
-  // loading them again, even at 16 items a thread)
+  // bounding-box centre, then the keys, the coordinates kept in registers
+  // between the two (measured faster than loading them again, even at 16
+  // items a thread)
   float px[IPT], py[IPT], pz[IPT];
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
 #pragma unroll
   for (int j = 0; j < IPT; j++) {
-    const int i = j * BLOCK + t;
+    const int i = (wv * IPT + j) * 64 + ln;
     if (i < n) {
       const float p[3] = {x[o + i], y[o + i], z[o + i]};
       px[j] = p[0];
@@ -447,7 +447,7 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_body(
   uint32_t key[IPT];
 #pragma unroll
   for (int j = 0; j < IPT; j++) {
-    const int i = j * BLOCK + t;
+    const int i = (wv * IPT + j) * 64 + ln;
     key[j] = 0xffffffffu;  // padding sorts after every vertex (stable: after equal keys)
     if (i < n) {
       key[j] = direction_key<kGridBitsSmall>(px[j] - c[0], py[j] - c[1], pz[j] - c[2])
@@ -455,13 +455,9 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_body(
                (uint32_t)i;
     }
   }
-  // blocked order (thread t: items t*IPT ...) is index order, which a stable
-  // sort keeps among equal keys, as the segmented sort does
-  if constexpr (IPT > 1) {
-    Exchange(tmp.ex).StripedToBlocked(key);
-    __syncthreads();
-  }
-  Sort(tmp.sort).SortBlockedToStriped(key, kIndexBits, kIndexBits + 2 * kGridBitsSmall);
+  // a stable sort keeps index order among equal keys, as the segmented sort does
+  Sort().sort_warp_striped_to_striped(key, sort_storage, kIndexBits,
+                                      kIndexBits + 2 * kGridBitsSmall);
 
   // rank j*BLOCK + t: the copy (coalesced) and, per 32-lane slice, one cluster
   const int padded = (n + kCluster - 1) / kCluster * kCluster;
