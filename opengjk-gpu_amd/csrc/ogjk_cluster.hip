@@ -212,75 +212,6 @@ __global__ void __launch_bounds__(256) k_order_keys(const float* __restrict__ x,
   }
 }
 
-// The batch's sorted positions, one thread each: position p of the batch's
-// concatenated segments is vertex i = p - kbeg[q] of body blist[b0 + q]
-// (q by binary search); its slot gets the vertex sorted there.  Blocks are
-// remapped so each XCD (blocks i, i + 8, ...) takes one contiguous eighth of
-// the positions: an XCD then gathers from about one body at a time, whose
-// vertices stay in its L2 (one block per body gathered from many bodies at
-// once and missed in L2 on nearly every load).
-__global__ void __launch_bounds__(256) k_cluster_scatter(
-    const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
-    const int64_t* __restrict__ off, const int64_t* __restrict__ slot,
-    const int* __restrict__ blist, int b0, int nb, const int* __restrict__ kbeg, int nv,
-    const int* __restrict__ sorted, float* __restrict__ cx, float* __restrict__ cy,
-    float* __restrict__ cz, int32_t* __restrict__ orig) {
-  // the grid is a multiple of 8 blocks, so this is a bijection of [0, grid)
-  const unsigned per = gridDim.x / 8u;
-  const unsigned lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
-  const int64_t p0 = (int64_t)lb * 256;
-  // last q with kbeg[q] <= pos, searched in [lo, nb - 1] (kbeg[lo] <= pos)
-  auto last_start = [&](int lo, int64_t pos) {
-    int hi = nb - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (kbeg[mid] <= pos) lo = mid;
-      else hi = mid - 1;
-    }
-    return lo;
-  };
-  // one search for the block's first position; a block rarely spans more
-  // than one body, so each thread steps on from there (a few steps, then a
-  // search when the block crosses many small bodies)
-  __shared__ int q_first;
-  if (threadIdx.x == 0) q_first = last_start(0, p0);
-  __syncthreads();
-  const int64_t p = p0 + threadIdx.x;
-  if (p >= nv) return;
-  int q = q_first;
-  for (int k = 0; k < 4 && q + 1 < nb && kbeg[q + 1] <= p; k++) q++;
-  if (q + 1 < nb && kbeg[q + 1] <= p) q = last_start(q + 1, p);
-  const int b = blist[b0 + q];
-  const int i = (int)(p - kbeg[q]);
-  const int l = sorted[p];
-  const int64_t o = off[b], s = slot[b] + i;
-  cx[s] = x[o + l];
-  cy[s] = y[o + l];
-  cz[s] = z[o + l];
-  orig[s] = l;
-}
-
-// The padding slots of a body, up to the next multiple of kCluster: zeros and
-// orig = -1 (one thread per body of the batch).
-__global__ void __launch_bounds__(256) k_cluster_pad(const int32_t* __restrict__ cnt,
-                                                     const int64_t* __restrict__ slot,
-                                                     const int* __restrict__ blist, int b0, int nb,
-                                                     float* __restrict__ cx, float* __restrict__ cy,
-                                                     float* __restrict__ cz,
-                                                     int32_t* __restrict__ orig) {
-  const int q = blockIdx.x * 256 + threadIdx.x;
-  if (q >= nb) return;
-  const int b = blist[b0 + q];
-  const int n = cnt[b];
-  const int padded = (n + kCluster - 1) / kCluster * kCluster;
-  for (int i = n; i < padded; i++) {
-    cx[slot[b] + i] = 0.0f;
-    cy[slot[b] + i] = 0.0f;
-    cz[slot[b] + i] = 0.0f;
-    orig[slot[b] + i] = -1;
-  }
-}
-
 // The value of lane CTRL-permuted within each 16-lane row (DPP: CTRL 0xB1
 // pairs lanes i and i ^ 1, 0x4E lanes i and i ^ 2, 0x141 mirrors each 8-lane
 // half, 0x140 each row).
@@ -396,7 +327,7 @@ OGJK_DEV void block_box(float (&lo)[3], float (&hi)[3], float* red) {
 // its whole part of the cluster copy -- the same keys as k_order_keys, a
 // stable sort in LDS, the copy in sorted order and every cluster's bound --
 // which is exactly what the batched path (keys, device segmented sort,
-// scatter, k_cluster_bounds) produces, in one pass over the body.  Small
+// scatter with bounds) produces, in one pass over the body.  Small
 // bodies take one-wave blocks (no block barriers in the reductions).
 template <int BLOCK, int IPT>
 __global__ void __launch_bounds__(BLOCK) k_cluster_body(
@@ -480,21 +411,73 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_body(
   }
 }
 
-// One kCluster-lane slice per cluster (the batched path's bounds).
-__global__ void __launch_bounds__(256) k_cluster_bounds(const float* __restrict__ cx,
-                                                        const float* __restrict__ cy,
-                                                        const float* __restrict__ cz,
-                                                        const int32_t* __restrict__ orig,
-                                                        int64_t c0, int64_t n_clusters,
-                                                        ClusterBound* __restrict__ bounds) {
-  const int64_t c = c0 + ((int64_t)blockIdx.x * 256 + threadIdx.x) / kCluster;
-  const int t = threadIdx.x & (kCluster - 1);
-  const bool in = c < n_clusters;
-  const int64_t s = c * kCluster + t;
-  const bool valid = in && orig[s] >= 0;
-  const float p[3] = {valid ? cx[s] : 0.0f, valid ? cy[s] : 0.0f, valid ? cz[s] : 0.0f};
-  const ClusterBound cb = slice_bound(valid, p);
-  if (in && t == 0) bounds[c] = cb;
+// The batch's padded slots, one thread each.  Padded position p of the batch
+// (its bodies blist[b0 + q] in order, each padded to a multiple of kCluster;
+// body q from pbeg[q], q by search) is slot i = p - pbeg[q] of that body: it
+// gets the vertex sorted to the body's rank i (segment q of `sorted` starts
+// at kbeg[q]), or padding (zeros, orig -1) past the body's count.  Bodies
+// start on cluster boundaries, so each 32-lane slice holds one whole cluster
+// and writes its bound as well.  Blocks are remapped so each XCD (blocks i,
+// i + 8, ...) takes one contiguous eighth of the positions: an XCD then
+// gathers from about one body at a time, whose vertices stay in its L2 (one
+// block per body gathered from many bodies at once and missed in L2 on nearly
+// every load).
+__global__ void __launch_bounds__(256) k_cluster_scatter(
+    const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
+    const int64_t* __restrict__ off, const int64_t* __restrict__ slot,
+    const int* __restrict__ blist, int b0, int nb, const int* __restrict__ kbeg,
+    const int* __restrict__ pbeg, int ptotal, const int* __restrict__ sorted,
+    float* __restrict__ cx, float* __restrict__ cy, float* __restrict__ cz,
+    int32_t* __restrict__ orig, ClusterBound* __restrict__ bounds) {
+  // the grid is a multiple of 8 blocks, so this is a bijection of [0, grid)
+  const unsigned per = gridDim.x / 8u;
+  const unsigned lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  const int64_t p0 = (int64_t)lb * 256;
+  // last q with pbeg[q] <= pos, searched in [lo, nb - 1] (pbeg[lo] <= pos)
+  auto last_start = [&](int lo, int64_t pos) {
+    int hi = nb - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (pbeg[mid] <= pos) lo = mid;
+      else hi = mid - 1;
+    }
+    return lo;
+  };
+  // one search for the block's first position; a block rarely spans more
+  // than one body, so each thread steps on from there (a few steps, then a
+  // search when the block crosses many small bodies)
+  __shared__ int q_first;
+  if (threadIdx.x == 0) q_first = last_start(0, p0);
+  __syncthreads();
+  const int64_t p = p0 + threadIdx.x;
+  const bool in = p < ptotal;  // ptotal is a multiple of kCluster: whole slices in or out
+  bool valid = false;
+  float v[3] = {0.0f, 0.0f, 0.0f};
+  int64_t dst = 0;
+  if (in) {
+    int q = q_first;
+    for (int k = 0; k < 4 && q + 1 < nb && pbeg[q + 1] <= p; k++) q++;
+    if (q + 1 < nb && pbeg[q + 1] <= p) q = last_start(q + 1, p);
+    const int b = blist[b0 + q];
+    const int i = (int)(p - pbeg[q]);
+    valid = i < kbeg[q + 1] - kbeg[q];
+    int l = -1;
+    if (valid) {
+      l = sorted[kbeg[q] + i];
+      const int64_t o = off[b];
+      v[0] = x[o + l];
+      v[1] = y[o + l];
+      v[2] = z[o + l];
+    }
+    dst = slot[b] + i;
+    cx[dst] = v[0];
+    cy[dst] = v[1];
+    cz[dst] = v[2];
+    orig[dst] = l;
+  }
+  // every lane reaches the slice's reduction
+  const ClusterBound cb = slice_bound(valid, v);
+  if (in && (threadIdx.x & (kCluster - 1)) == 0) bounds[dst / kCluster] = cb;
 }
 
 
@@ -583,7 +566,7 @@ int64_t cluster_slots(int64_t n_verts, int32_t n_bodies) {
 // others get body_slot -1 and no slots (queries sweep them whole).  The slot
 // layout (a scan of the padded sizes) and the per-class body lists are made
 // on the device; only the batched bodies (more than lds_max vertices) come
-// back to the host, which cuts them into batches of at most kBatch vertices
+// back to the host, which cuts them into batches of at most kBatch padded slots
 // so the sort's scratch stays bounded.
 hipError_t build_clusters(const float* x, const float* y, const float* z, const int64_t* off,
                           const int32_t* cnt, int32_t n_bodies, float* cx, float* cy, float* cz,
@@ -603,7 +586,8 @@ hipError_t build_clusters(const float* x, const float* y, const float* z, const 
     lds_max = std::max(0, std::min(atoi(v), kLdsBodyMax));
 
   uint32_t *keys = nullptr, *keys2 = nullptr;
-  int *vals = nullptr, *vals2 = nullptr, *kbeg = nullptr, *blist = nullptr, *lbig = nullptr;
+  int *vals = nullptr, *vals2 = nullptr, *kbeg = nullptr, *pbeg = nullptr, *blist = nullptr,
+      *lbig = nullptr;
   unsigned* ccount = nullptr;
   int64_t *psize = nullptr, *pos = nullptr, *dgath = nullptr;
   void* tmp = nullptr;
@@ -614,6 +598,7 @@ hipError_t build_clusters(const float* x, const float* y, const float* z, const 
     (void)hipFree(vals);
     (void)hipFree(vals2);
     (void)hipFree(kbeg);
+    (void)hipFree(pbeg);
     (void)hipFree(blist);
     (void)hipFree(lbig);
     (void)hipFree(ccount);
@@ -735,16 +720,19 @@ hipError_t build_clusters(const float* x, const float* y, const float* z, const 
   std::vector<int> kbeg_h;  // batch-local key offset of every batched body, batch after batch
   std::vector<int64_t> bverts;
   {
-    int64_t acc = 0;
+    // a batch holds at most kBatch padded slots (so at most kBatch vertices),
+    // or one body alone
+    int64_t acc = 0, pacc = 0;  // vertices, padded slots
     for (int q = 0; q < n_large; q++) {
-      const int64_t n = lcs[2 * q];
-      if (acc + n > kBatch) {
+      const int64_t n = lcs[2 * q], pn = (n + kCluster - 1) / kCluster * kCluster;
+      if (pacc > 0 && pacc + pn > kBatch) {
         bstart.push_back(q);
         bverts.push_back(acc);
-        acc = 0;
+        acc = pacc = 0;
       }
       kbeg_h.push_back((int)acc);
       acc += n;
+      pacc += pn;
     }
     bverts.push_back(acc);
     bstart.push_back(n_large);
@@ -759,16 +747,23 @@ hipError_t build_clusters(const float* x, const float* y, const float* z, const 
   if (!large.empty() &&
       ((e = hipMalloc(&keys, 4 * maxv)) != hipSuccess || (e = hipMalloc(&keys2, 4 * maxv)) ||
        (e = hipMalloc(&vals, 4 * maxv)) || (e = hipMalloc(&vals2, 4 * maxv)) ||
-       (e = hipMalloc(&kbeg, sizeof(int) * ((size_t)maxb + 1)))))
+       (e = hipMalloc(&kbeg, sizeof(int) * ((size_t)maxb + 1))) ||
+       (e = hipMalloc(&pbeg, sizeof(int) * ((size_t)maxb + 1)))))
     return fail_with(e);
   for (size_t i = 0; i + 1 < bstart.size() && !large.empty(); i++) {
     const int b0 = bstart[i], nb = bstart[i + 1] - b0;
     const int nv = (int)bverts[i];
     if (nb == 0) continue;
-    // segment offsets: kbeg[0..nb], kbeg[nb] = nv (end of the last body)
+    // segment offsets: kbeg[0..nb], kbeg[nb] = nv (end of the last body); and
+    // the same over the bodies padded to whole clusters, pbeg
     std::vector<int> seg(kbeg_h.begin() + b0, kbeg_h.begin() + b0 + nb);
     seg.push_back(nv);
+    std::vector<int> pseg(nb + 1, 0);
+    for (int q = 0; q < nb; q++)
+      pseg[q + 1] = pseg[q] + (int)((lcs[2 * (b0 + q)] + kCluster - 1) / kCluster * kCluster);
     if ((e = hipMemcpyAsync(kbeg, seg.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice,
+                            st)) != hipSuccess ||
+        (e = hipMemcpyAsync(pbeg, pseg.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice,
                             st)) != hipSuccess) {
       cleanup();
       return e;
@@ -797,25 +792,16 @@ hipError_t build_clusters(const float* x, const float* y, const float* z, const 
       cleanup();
       return e;
     }
-    const unsigned sblocks = ((unsigned)((nv + 255) / 256) + 7u) / 8u * 8u;  // a multiple of 8
+    const int ptotal = pseg[nb];
+    const unsigned sblocks = ((unsigned)((ptotal + 255) / 256) + 7u) / 8u * 8u;  // a multiple of 8
     hipLaunchKernelGGL(k_cluster_scatter, dim3(sblocks), dim3(256), 0, st, x, y, z, off, body_slot,
-                       lbig, b0, nb, kbeg, nv, vals2, cx, cy, cz, orig);
-    hipLaunchKernelGGL(k_cluster_pad, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, cnt,
-                       body_slot, lbig, b0, nb, cx, cy, cz, orig);
+                       lbig, b0, nb, kbeg, pbeg, ptotal, vals2, cx, cy, cz, orig,
+                       (ClusterBound*)bounds);
     // the host vectors of this batch die at the end of the iteration
     if ((e = hipStreamSynchronize(st)) != hipSuccess) {
       cleanup();
       return e;
     }
-  }
-  const int64_t n_clusters = total / kCluster;
-  // bounds of the batched bodies' clusters (recomputed for every cluster: the
-  // blocks of k_cluster_body give the same values); 2^24 clusters (2^29
-  // work-items) per dispatch: a dispatch holds < 2^32
-  for (int64_t c0 = 0; c0 < n_clusters && !large.empty(); c0 += (int64_t)1 << 24) {
-    const int64_t nc = n_clusters - c0 < ((int64_t)1 << 24) ? n_clusters - c0 : ((int64_t)1 << 24);
-    hipLaunchKernelGGL(k_cluster_bounds, dim3((unsigned)((nc * kCluster + 255) / 256)), dim3(256),
-                       0, st, cx, cy, cz, orig, c0, n_clusters, (ClusterBound*)bounds);
   }
   hipLaunchKernelGGL(k_mark_super, dim3((unsigned)((n_bodies + 255) / 256)), dim3(256), 0, st, cnt,
                      body_slot, body_super, n_bodies, (ClusterBound*)bounds);
