@@ -295,10 +295,12 @@ class DeviceScene:
         dev = self.x.device
         slots = int(lib().ogjk_cluster_slots(self.n_verts, self.n_bodies))
         ns = max(slots, 32)
-        self.cx = t.empty(ns, dtype=t.float32, device=dev)
-        self.cy = t.empty(ns, dtype=t.float32, device=dev)
-        self.cz = t.empty(ns, dtype=t.float32, device=dev)
-        self.corig = t.empty(ns, dtype=t.int32, device=dev)
+        # x, y, z and orig as views of one allocation (one large device
+        # allocation took a third of the time of four), each 256-byte aligned
+        stride = -(-ns // 64) * 64
+        buf = t.empty(4 * stride, dtype=t.float32, device=dev)
+        self.cx, self.cy, self.cz = (buf[k * stride:k * stride + ns] for k in range(3))
+        self.corig = buf[3 * stride:3 * stride + ns].view(t.int32)
         self.cslot = t.empty(max(self.n_bodies, 1), dtype=t.int64, device=dev)
         self.cbounds = t.empty((ns // 32 + 1) * 8, dtype=t.float32, device=dev)
         nsup = int(lib().ogjk_cluster_supers(ns))
